@@ -1,0 +1,508 @@
+// dg_api.cpp -- C ABI (include/deepgreen.h) over the fused HIP step.
+//
+// A context owns two interior-only SoA conserved buffers on the device (the
+// state ping-pongs between them), the per-workgroup partial sums, the
+// device-resident scalars (dt, run clock, conservation sums) and one HIP
+// stream.  Host arrays crossing the ABI use the reference's ghosted layout
+// (grid.rs:79-109); the device never stores ghosts: the kernels map ghost
+// positions through the boundary conditions.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/deepgreen.h"
+#include "dg_internal.h"
+
+struct dg_ctx {
+    dg_config cfg{};
+    int prec = 1, fast = 0, tile_y = 8, chunk_x = 8;
+    dg::StepArgs args{};
+    size_t n3 = 0, el = 4;
+    void *buf[2][5] = {};
+    int cur = 0;
+    double *partials = nullptr;
+    int nparts_reduce = 0;
+    dg::Scalars *sc = nullptr;      // device
+    dg::Scalars *sc_host = nullptr; // pinned mirror
+    void *stage = nullptr;          // pinned staging (one field)
+    void *scratch = nullptr;        // device scratch (one state) for primitive downloads
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    bool have_state = false;
+    std::string err;
+};
+
+namespace {
+
+const int kGraphSteps = 16;  // even: the buffer parity is unchanged by a replay
+
+int fail(dg_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(dg_ctx *c, hipError_t e, const char *what)
+{
+    return fail(c, DG_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(c, expr)                                                                \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) return hip_fail((c), e_, #expr);                         \
+    } while (0)
+
+__global__ void arm_kernel(dg::Scalars *sc, double t_end, long long max_steps_abs, int reset)
+{
+    if (reset) { sc->t = 0.0; sc->steps = 0; }
+    sc->t_end = t_end;
+    sc->max_steps = max_steps_abs;
+    sc->active = !((sc->t > sc->t_end) || (sc->steps >= sc->max_steps));
+}
+
+int launch_one_step(dg_ctx *c)
+{
+    const void *in[5];
+    void *out[5];
+    for (int f = 0; f < 5; ++f) { in[f] = c->buf[c->cur][f]; out[f] = c->buf[1 - c->cur][f]; }
+    int rc = dg::launch_step(c->prec, c->fast, c->tile_y, c->args, in, out, c->sc, c->partials, c->stream);
+    if (rc) return fail(c, DG_ERR_DEVICE, std::string("step launch: ") + hipGetErrorString(hipGetLastError()));
+    rc = dg::launch_finalize(c->prec, c->args, c->sc, c->partials, c->args.nblocks, c->cfg.cfl,
+                             c->cfg.dt_nan_fallback, 1, c->stream);
+    if (rc) return fail(c, DG_ERR_DEVICE, "finalize launch");
+    c->cur = 1 - c->cur;
+    return DG_OK;
+}
+
+int reduce_current(dg_ctx *c)
+{
+    const void *u[5];
+    for (int f = 0; f < 5; ++f) u[f] = c->buf[c->cur][f];
+    if (dg::launch_reduce(c->prec, c->args, u, c->partials, c->nparts_reduce, c->stream))
+        return fail(c, DG_ERR_DEVICE, "reduce launch");
+    if (dg::launch_finalize(c->prec, c->args, c->sc, c->partials, c->nparts_reduce, c->cfg.cfl,
+                            c->cfg.dt_nan_fallback, 0, c->stream))
+        return fail(c, DG_ERR_DEVICE, "finalize launch");
+    return DG_OK;
+}
+
+int read_scalars(dg_ctx *c)
+{
+    HIPCHK(c, hipMemcpyAsync(c->sc_host, c->sc, sizeof(dg::Scalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DG_OK;
+}
+
+// Boundary map of one axis on the host (same rules as the kernels):
+// ghosted index gi (0..n+3) -> interior index and flip.
+inline void host_map(int64_t gi, int64_t n, int lo, int hi, int64_t *src, int *flip)
+{
+    int64_t p = gi - 2;
+    *flip = 0;
+    if (p < 0) {
+        if (lo == DG_PERIODIC) *src = p + n;
+        else if (lo == DG_OUTFLOW) *src = 0;
+        else { *src = -1 - p; *flip = 1; }
+    } else if (p >= n) {
+        if (hi == DG_PERIODIC) *src = p - n;
+        else if (hi == DG_OUTFLOW) *src = n - 1;
+        else { *src = 2 * n - 1 - p; *flip = 1; }
+    } else {
+        *src = p;
+    }
+}
+
+template <typename T>
+void scatter_ghosted(const dg_ctx *c, const std::vector<const T *> &interior, T *const out[5])
+{
+    const int64_t nx = c->cfg.n[0], ny = c->cfg.n[1], nz = c->cfg.n[2];
+    const int64_t gy = ny + 4, gz = nz + 4;
+    std::vector<int64_t> sy(gy), sz(gz);
+    std::vector<int> fy(gy), fz(gz);
+    for (int64_t j = 0; j < gy; ++j) host_map(j, ny, c->cfg.bc[2], c->cfg.bc[3], &sy[j], &fy[j]);
+    for (int64_t k = 0; k < gz; ++k) host_map(k, nz, c->cfg.bc[4], c->cfg.bc[5], &sz[k], &fz[k]);
+    for (int64_t i = 0; i < nx + 4; ++i) {
+        int64_t sx; int fx;
+        host_map(i, nx, c->cfg.bc[0], c->cfg.bc[1], &sx, &fx);
+        for (int64_t j = 0; j < gy; ++j)
+            for (int64_t k = 0; k < gz; ++k) {
+                const int64_t s = (sx * ny + sy[j]) * nz + sz[k];
+                const int64_t d = (i * gy + j) * gz + k;
+                for (int f = 0; f < 5; ++f) {
+                    T v = interior[f][s];
+                    if ((f == 1 && fx) || (f == 2 && fy[j]) || (f == 3 && fz[k])) v = -v;
+                    out[f][d] = v;
+                }
+            }
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+void dg_config_default(dg_config *cfg)
+{
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->n[0] = cfg->n[1] = cfg->n[2] = 64;
+    cfg->nghosts = 2;
+    for (int i = 0; i < 6; ++i) cfg->bc[i] = DG_PERIODIC;
+    cfg->ds = 1.0 / 64.0;
+    cfg->gamma = 5.0 / 3.0;
+    cfg->cfl = 0.2;
+    cfg->g = -0.1;
+    cfg->gravity = 0;
+    cfg->precision = DG_FP32;
+    cfg->math = DG_MATH_EXACT;
+    cfg->limiter = DG_LIMITER_NONE;
+    cfg->dt_nan_fallback = 1;
+}
+
+const char *dg_last_error(const dg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int dg_device_count(int *count)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    if (count) *count = n;
+    return DG_OK;
+}
+
+int dg_create(const dg_config *cfg, dg_ctx **out)
+{
+    if (!cfg || !out) return DG_ERR_INVALID;
+    *out = nullptr;
+    if (cfg->nghosts != 2) return DG_ERR_INVALID;
+    for (int d = 0; d < 3; ++d)
+        if (cfg->n[d] < 2 || cfg->n[d] > (1 << 20)) return DG_ERR_INVALID;
+    for (int i = 0; i < 6; ++i)
+        if (cfg->bc[i] < 0 || cfg->bc[i] > 2) return DG_ERR_INVALID;
+    for (int d = 0; d < 3; ++d)  // periodic faces pair up (grid.rs:55-63)
+        if ((cfg->bc[2 * d] == DG_PERIODIC) != (cfg->bc[2 * d + 1] == DG_PERIODIC)) return DG_ERR_INVALID;
+    if (!(cfg->ds > 0) || !(cfg->gamma > 1) || !(cfg->cfl > 0)) return DG_ERR_INVALID;
+    if (cfg->precision < 0 || cfg->precision > 2 || cfg->math < 0 || cfg->math > 1) return DG_ERR_INVALID;
+    if (cfg->limiter < 0 || cfg->limiter > 2) return DG_ERR_INVALID;
+    if (cfg->precision == DG_FP64 && cfg->math == DG_MATH_FAST) return DG_ERR_INVALID;
+    const size_t n3 = (size_t)cfg->n[0] * cfg->n[1] * cfg->n[2];
+    if (n3 > ((size_t)1 << 34)) return DG_ERR_INVALID;
+
+    dg_ctx *c = new (std::nothrow) dg_ctx();
+    if (!c) return DG_ERR_ALLOC;
+    c->cfg = *cfg;
+    c->prec = cfg->precision;
+    c->fast = cfg->math == DG_MATH_FAST;
+    c->el = c->prec == DG_FP64 ? sizeof(double) : sizeof(float);
+    c->n3 = n3;
+    int ty = cfg->tile_y ? cfg->tile_y : 8;
+    if (c->prec == DG_FP64 && ty > 8) ty = 8;
+    if (ty != 4 && ty != 8 && ty != 16) { delete c; return DG_ERR_INVALID; }
+    c->tile_y = ty;
+
+    dg::StepArgs &a = c->args;
+    for (int d = 0; d < 3; ++d) a.n[d] = (int)cfg->n[d];
+    for (int i = 0; i < 6; ++i) a.bc[i] = cfg->bc[i];
+    a.limiter = cfg->limiter;
+    a.gravity = cfg->gravity;
+    a.ds = cfg->ds;
+    a.gamma = cfg->gamma;
+    a.g = cfg->g;
+    a.nby = (a.n[1] + ty - 1) / ty;
+    a.nbz = (a.n[2] + 63) / 64;
+    int lx = cfg->chunk_x;
+    if (lx <= 0) {
+        const long long yz = (long long)a.nby * a.nbz;
+        long long want = (long long)a.n[0] * yz / 2048;
+        lx = (int)(want < 8 ? 8 : want);
+    }
+    if (lx > a.n[0]) lx = a.n[0];
+    a.lx = lx;
+    c->chunk_x = lx;
+    a.nbx = (a.n[0] + lx - 1) / lx;
+    const long long nb = (long long)a.nbx * a.nby * a.nbz;
+    if (nb > INT_MAX / 2) { delete c; return DG_ERR_INVALID; }
+    a.nblocks = (int)nb;
+    a.swizzle = (a.nblocks % 8 == 0) ? 1 : 0;
+    c->nparts_reduce = dg::reduce_blocks(a);
+
+    hipError_t e = hipSetDevice(cfg->device);
+    if (e != hipSuccess) { delete c; return DG_ERR_DEVICE; }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return DG_ERR_DEVICE; }
+    int rc = DG_OK;
+    for (int b = 0; b < 2 && rc == DG_OK; ++b)
+        for (int f = 0; f < 5 && rc == DG_OK; ++f)
+            if (hipMalloc(&c->buf[b][f], n3 * c->el) != hipSuccess) rc = DG_ERR_ALLOC;
+    const size_t np = (size_t)std::max(a.nblocks, c->nparts_reduce);
+    if (rc == DG_OK && hipMalloc((void **)&c->partials, np * dg::PARTIAL_STRIDE * sizeof(double)) != hipSuccess)
+        rc = DG_ERR_ALLOC;
+    if (rc == DG_OK && hipMalloc((void **)&c->sc, sizeof(dg::Scalars)) != hipSuccess) rc = DG_ERR_ALLOC;
+    if (rc == DG_OK && hipHostMalloc((void **)&c->sc_host, sizeof(dg::Scalars), hipHostMallocDefault) != hipSuccess)
+        rc = DG_ERR_ALLOC;
+    if (rc == DG_OK && hipHostMalloc(&c->stage, n3 * c->el, hipHostMallocDefault) != hipSuccess) rc = DG_ERR_ALLOC;
+    if (rc != DG_OK) {
+        dg_destroy(c);
+        return rc;
+    }
+    std::memset(c->sc_host, 0, sizeof(dg::Scalars));
+    c->sc_host->t_end = INFINITY;
+    c->sc_host->max_steps = LLONG_MAX;
+    c->sc_host->active = 1;
+    if (hipMemcpy(c->sc, c->sc_host, sizeof(dg::Scalars), hipMemcpyHostToDevice) != hipSuccess) {
+        dg_destroy(c);
+        return DG_ERR_DEVICE;
+    }
+    *out = c;
+    return DG_OK;
+}
+
+void dg_destroy(dg_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    for (int b = 0; b < 2; ++b)
+        for (int f = 0; f < 5; ++f)
+            if (c->buf[b][f]) hipFree(c->buf[b][f]);
+    if (c->partials) hipFree(c->partials);
+    if (c->sc) hipFree(c->sc);
+    if (c->scratch) hipFree(c->scratch);
+    if (c->sc_host) hipHostFree(c->sc_host);
+    if (c->stage) hipHostFree(c->stage);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int dg_set_state(dg_ctx *c, const void *const f[5], int kind)
+{
+    if (!c || !f || (kind != DG_PRIMITIVE && kind != DG_CONSERVED)) return fail(c, DG_ERR_INVALID, "bad argument");
+    for (int q = 0; q < 5; ++q)
+        if (!f[q]) return fail(c, DG_ERR_INVALID, "null field");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const int64_t nx = c->cfg.n[0], ny = c->cfg.n[1], nz = c->cfg.n[2];
+    const int64_t gy = ny + 4, gz = nz + 4;
+    for (int q = 0; q < 5; ++q) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));  // staging buffer reuse
+        const char *src = static_cast<const char *>(f[q]);
+        char *dst = static_cast<char *>(c->stage);
+        for (int64_t i = 0; i < nx; ++i)
+            for (int64_t j = 0; j < ny; ++j) {
+                const int64_t s = ((i + 2) * gy + (j + 2)) * gz + 2;
+                const int64_t d = (i * ny + j) * nz;
+                std::memcpy(dst + d * c->el, src + s * c->el, nz * c->el);
+            }
+        HIPCHK(c, hipMemcpyAsync(c->buf[c->cur][q], c->stage, c->n3 * c->el, hipMemcpyHostToDevice, c->stream));
+    }
+    if (kind == DG_PRIMITIVE) {
+        void *u[5];
+        for (int q = 0; q < 5; ++q) u[q] = c->buf[c->cur][q];
+        if (dg::launch_prim_to_cons(c->prec, c->args, u, c->stream)) return fail(c, DG_ERR_DEVICE, "prim_to_cons");
+    }
+    int rc = reduce_current(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    return DG_OK;
+}
+
+int dg_get_state(dg_ctx *c, void *const f[5], int kind)
+{
+    if (!c || !f || (kind != DG_PRIMITIVE && kind != DG_CONSERVED)) return fail(c, DG_ERR_INVALID, "bad argument");
+    if (!c->have_state) return fail(c, DG_ERR_STATE, "no state set");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const void *const *srcs = (const void *const *)c->buf[c->cur];
+    void *prim[5];
+    if (kind == DG_PRIMITIVE) {
+        if (!c->scratch) HIPCHK(c, hipMalloc(&c->scratch, 5 * c->n3 * c->el));
+        for (int q = 0; q < 5; ++q) prim[q] = static_cast<char *>(c->scratch) + q * c->n3 * c->el;
+        if (dg::launch_cons_to_prim(c->prec, c->args, srcs, prim, c->stream))
+            return fail(c, DG_ERR_DEVICE, "cons_to_prim");
+        srcs = prim;
+    }
+    std::vector<char> host(5 * c->n3 * c->el);
+    for (int q = 0; q < 5; ++q)
+        HIPCHK(c, hipMemcpyAsync(host.data() + q * c->n3 * c->el, srcs[q], c->n3 * c->el, hipMemcpyDeviceToHost,
+                                 c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->prec == DG_FP64) {
+        std::vector<const double *> in(5);
+        for (int q = 0; q < 5; ++q) in[q] = reinterpret_cast<const double *>(host.data() + q * c->n3 * c->el);
+        scatter_ghosted<double>(c, in, reinterpret_cast<double *const *>(f));
+    } else {
+        std::vector<const float *> in(5);
+        for (int q = 0; q < 5; ++q) in[q] = reinterpret_cast<const float *>(host.data() + q * c->n3 * c->el);
+        scatter_ghosted<float>(c, in, reinterpret_cast<float *const *>(f));
+    }
+    return DG_OK;
+}
+
+int dg_step(dg_ctx *c, double *dt_used)
+{
+    if (!c) return DG_ERR_INVALID;
+    if (!c->have_state) return fail(c, DG_ERR_STATE, "no state set");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    hipLaunchKernelGGL(arm_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, (double)INFINITY, (long long)LLONG_MAX, 0);
+    int rc = read_scalars(c);
+    if (rc) return rc;
+    const double dt = c->sc_host->dt;
+    if (!c->cfg.dt_nan_fallback && !std::isfinite(dt)) return fail(c, DG_ERR_NONFINITE, "dt is not finite");
+    rc = launch_one_step(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (dt_used) *dt_used = dt;
+    return DG_OK;
+}
+
+int dg_reset_clock(dg_ctx *c)
+{
+    if (!c) return DG_ERR_INVALID;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    hipLaunchKernelGGL(arm_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, (double)INFINITY, (long long)LLONG_MAX, 1);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DG_OK;
+}
+
+static int ensure_graph(dg_ctx *c)
+{
+    if (c->graph) return DG_OK;
+    hipGraph_t g = nullptr;
+    const int cur0 = c->cur;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = DG_OK;
+    for (int s = 0; s < kGraphSteps && rc == DG_OK; ++s) rc = launch_one_step(c);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->cur = cur0;  // capture launched nothing
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) { c->graph = nullptr; return hip_fail(c, e, "hipGraphInstantiate"); }
+    c->graph_steps = kGraphSteps;
+    return DG_OK;
+}
+
+int dg_run(dg_ctx *c, int64_t max_steps, double t_end, double *t_out, int64_t *steps_out)
+{
+    if (!c || max_steps < 0) return fail(c, DG_ERR_INVALID, "bad argument");
+    if (!c->have_state) return fail(c, DG_ERR_STATE, "no state set");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    int rc = read_scalars(c);
+    if (rc) return rc;
+    const long long start = c->sc_host->steps;
+    const long long cap = (max_steps > LLONG_MAX - start) ? LLONG_MAX : start + max_steps;
+    hipLaunchKernelGGL(arm_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, t_end, cap, 0);
+    rc = ensure_graph(c);
+    if (rc) return rc;
+    const bool timed = std::isfinite(t_end);
+    int64_t left = max_steps;
+    while (left > 0) {
+        if (left >= c->graph_steps) {
+            HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+            left -= c->graph_steps;
+        } else {
+            rc = launch_one_step(c);
+            if (rc) return rc;
+            left -= 1;
+        }
+        if (timed) {
+            rc = read_scalars(c);
+            if (rc) return rc;
+            if (!c->sc_host->active) break;
+        }
+    }
+    rc = read_scalars(c);
+    if (rc) return rc;
+    if (t_out) *t_out = c->sc_host->t;
+    if (steps_out) *steps_out = c->sc_host->steps;
+    if (!c->cfg.dt_nan_fallback && c->sc_host->nonfinite) return fail(c, DG_ERR_NONFINITE, "dt is not finite");
+    return DG_OK;
+}
+
+int dg_conservation_sums(dg_ctx *c, double out[5])
+{
+    if (!c || !out) return DG_ERR_INVALID;
+    if (!c->have_state) return fail(c, DG_ERR_STATE, "no state set");
+    int rc = read_scalars(c);
+    if (rc) return rc;
+    for (int q = 0; q < 5; ++q) out[q] = c->sc_host->sums[q];
+    return DG_OK;
+}
+
+int dg_next_dt(dg_ctx *c, double *dt)
+{
+    if (!c || !dt) return DG_ERR_INVALID;
+    if (!c->have_state) return fail(c, DG_ERR_STATE, "no state set");
+    int rc = read_scalars(c);
+    if (rc) return rc;
+    *dt = c->sc_host->dt;
+    return DG_OK;
+}
+
+int dg_device_state(dg_ctx *c, void *ptrs[5])
+{
+    if (!c || !ptrs) return DG_ERR_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 5; ++q) ptrs[q] = c->buf[c->cur][q];
+    return DG_OK;
+}
+
+int dg_time_steps(dg_ctx *c, int64_t nsteps, double *ms_total, double *ms_step_kernel, double *ms_reduce_kernel)
+{
+    if (!c || nsteps <= 0) return fail(c, DG_ERR_INVALID, "bad argument");
+    if (!c->have_state) return fail(c, DG_ERR_STATE, "no state set");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    hipLaunchKernelGGL(arm_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, (double)INFINITY, (long long)LLONG_MAX, 0);
+    std::vector<hipEvent_t> ev((size_t)nsteps * 2 + 1);
+    for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+    int rc = DG_OK;
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    for (int64_t s = 0; s < nsteps && rc == DG_OK; ++s) {
+        const void *in[5];
+        void *out[5];
+        for (int f = 0; f < 5; ++f) { in[f] = c->buf[c->cur][f]; out[f] = c->buf[1 - c->cur][f]; }
+        if (dg::launch_step(c->prec, c->fast, c->tile_y, c->args, in, out, c->sc, c->partials, c->stream))
+            rc = fail(c, DG_ERR_DEVICE, "step launch");
+        if (hipEventRecord(ev[2 * s + 1], c->stream) != hipSuccess) rc = DG_ERR_DEVICE;
+        if (dg::launch_finalize(c->prec, c->args, c->sc, c->partials, c->args.nblocks, c->cfg.cfl,
+                                c->cfg.dt_nan_fallback, 1, c->stream))
+            rc = fail(c, DG_ERR_DEVICE, "finalize launch");
+        if (hipEventRecord(ev[2 * s + 2], c->stream) != hipSuccess) rc = DG_ERR_DEVICE;
+        c->cur = 1 - c->cur;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) rc = DG_ERR_DEVICE;
+    double tk = 0, tr = 0;
+    float ms = 0;
+    for (int64_t s = 0; s < nsteps && rc == DG_OK; ++s) {
+        hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]);
+        tk += ms;
+        hipEventElapsedTime(&ms, ev[2 * s + 1], ev[2 * s + 2]);
+        tr += ms;
+    }
+    hipEventElapsedTime(&ms, ev[0], ev[2 * nsteps]);
+    for (auto &e : ev) hipEventDestroy(e);
+    if (rc) return rc;
+    if (ms_total) *ms_total = ms;
+    if (ms_step_kernel) *ms_step_kernel = tk / nsteps;
+    if (ms_reduce_kernel) *ms_reduce_kernel = tr / nsteps;
+    return DG_OK;
+}
+
+int dg_kernel_info(dg_ctx *c, int *tile_y, int *chunk_x, int64_t *workgroups, int *lds_bytes, int *threads)
+{
+    if (!c) return DG_ERR_INVALID;
+    if (tile_y) *tile_y = c->tile_y;
+    if (chunk_x) *chunk_x = c->chunk_x;
+    if (workgroups) *workgroups = c->args.nblocks;
+    if (lds_bytes) *lds_bytes = dg::step_lds_bytes(c->prec, c->tile_y);
+    if (threads) *threads = c->tile_y * 64;
+    return DG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,58 @@
+// dg_internal.h -- types shared by the HIP kernels (dg_step.hip) and the
+// C-ABI layer (dg_api.cpp).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dg {
+
+enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_WALL = 2 };
+enum : int { LIM_NONE = 0, LIM_MINMOD = 1, LIM_VANLEER = 2 };
+
+// Per-workgroup partial results written by the step and reduction kernels:
+// [0] max wave-speed denominator, [1] NaN seen (0/1), [2..6] conserved sums.
+constexpr int PARTIAL_STRIDE = 8;
+
+struct StepArgs {
+    int n[3];          // interior cells per axis
+    int bc[6];         // x-, x+, y-, y+, z-, z+
+    int limiter;
+    int gravity;
+    int lx;            // planes per workgroup along x
+    int nbx, nby, nbz; // workgroups per axis
+    int nblocks;
+    int swizzle;       // 1: spread tiles so each XCD works on a contiguous range
+    double ds, gamma, g;
+};
+
+// Device-resident scalars of a context.
+struct Scalars {
+    double dt;          // dt the next step uses (exactly representable in the state type)
+    double t;           // run clock (float arithmetic for DG_FP32)
+    long long steps;    // run step counter
+    double t_end;
+    long long max_steps;
+    int active;         // 1: the next step advances the state
+    int nonfinite;      // the last reduction saw a NaN wave speed
+    double sums[5];     // FP64 conservation sums of the current state
+    double maxden;      // max of sqrt(gamma p / rho) + |v| over the interior
+};
+
+template <typename T> struct Fields { T *f[5]; };
+
+// Kernel launch entry points (dg_step.hip).  `prec` is 0 FP64, 1 FP32,
+// 2 FP32_BULK; `fast` selects the reassociated arithmetic.
+int launch_step(int prec, int fast, int tile_y, const StepArgs &a, const void *const in[5],
+                void *const out[5], const Scalars *sc, double *partials, hipStream_t s);
+int launch_reduce(int prec, const StepArgs &a, const void *const u[5], double *partials,
+                  int nblocks, hipStream_t s);
+int launch_finalize(int prec, const StepArgs &a, Scalars *sc, const double *partials, int nparts,
+                    double cfl, int nan_fallback, int advance, hipStream_t s);
+int launch_prim_to_cons(int prec, const StepArgs &a, void *const u[5], hipStream_t s);
+int launch_cons_to_prim(int prec, const StepArgs &a, const void *const u[5], void *const w[5],
+                        hipStream_t s);
+int step_lds_bytes(int prec, int tile_y);
+int reduce_blocks(const StepArgs &a);
+
+} // namespace dg

@@ -1,0 +1,915 @@
+// dg_step.hip -- fused compressible-Euler step for gfx950 (MI355X).
+//
+// One kernel performs the whole reference step (physics.rs:595-681):
+// primitives (calc_primitives :109-141), ghost fill (boundaries.rs as index
+// maps), central or limited slopes (calc_gradients :198-215), the Hancock
+// half-step (extrapolate_primitives_t :217-282), the Rusanov face flux
+// (calc_flux_reconstruction :284-409, one evaluation per face), the
+// conservative update (add_fluxes :539-593) and, in its epilogue, the
+// wave-speed maximum for the next step's dt (calc_timestep :177-196) and the
+// FP64 conservation partial sums.  State is read once and written once.
+//
+// Work decomposition: a workgroup owns a (TY x 64) tile of the (y, z) plane
+// and marches along x over a chunk of planes.  Wave w is tile row w; lane l
+// is tile column l, so every global access is a 64-wide coalesced row.
+//   * primitives of three consecutive planes sit in an LDS ring (tile plus a
+//     two-cell halo), filled by mapping ghost positions through the boundary
+//     maps, so no ghost cells exist in HBM;
+//   * each thread evaluates slopes and the extrapolated state of its cell,
+//     then the flux through its +x (previous plane), +y and +z faces;
+//   * y faces exchange through LDS, z faces through lane shuffles, x faces
+//     through registers carried from the previous plane;
+//   * halo states (row -1, row TY, column -1, column 64) are evaluated by
+//     waves 0, 1 and 2.
+// Boundaries: a ghost state is evaluated at its mapped interior cell
+// (periodic: the wrapped cell; outflow: the boundary cell, which reproduces
+// the reference's copies of xtr and gradients, boundaries.rs:5-195); a wall
+// ghost state is the mirrored extrapolated state of the boundary cell plus
+// the normal slope evaluated on the mirrored ghost primitives
+// (handle_walls :197-407 applied to primitives and xtr, gradients computed on
+// the ghost).  The outflow lower face copies the first interior face
+// (apply_boundary_conditions4 on fluxes, boundaries.rs:107-111).
+//
+// Arithmetic: with FAST = false every expression keeps the reference's
+// operation order (the library is built with -ffp-contract=off and IEEE
+// division and square root), so results match the reference arithmetic bit
+// for bit.  FAST = true is the same scheme reassociated around FMAs and
+// hardware reciprocals.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "dg_internal.h"
+
+namespace dg {
+
+constexpr int TZ = 64;       // tile columns == wavefront width
+constexpr int RZ = TZ + 4;   // LDS region columns (two-cell halo)
+
+// ------------------------------------------------------------ math helpers --
+template <typename T> __device__ __forceinline__ T dsqrt(T x);
+template <> __device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
+template <> __device__ __forceinline__ double dsqrt<double>(double x) { return sqrt(x); }
+template <typename T> __device__ __forceinline__ T dmax(T a, T b);  // NaN-ignoring, as f32::max
+template <> __device__ __forceinline__ float dmax<float>(float a, float b) { return fmaxf(a, b); }
+template <> __device__ __forceinline__ double dmax<double>(double a, double b) { return fmax(a, b); }
+template <typename T> __device__ __forceinline__ T dabs(T x) { return x < T(0) ? -x : (x == T(0) ? T(0) * x : x); }
+template <> __device__ __forceinline__ float dabs<float>(float x) { return fabsf(x); }
+template <> __device__ __forceinline__ double dabs<double>(double x) { return fabs(x); }
+template <typename T> __device__ __forceinline__ T drcp(T x);
+template <> __device__ __forceinline__ float drcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double drcp<double>(double x) { return 1.0 / x; }
+template <typename T> __device__ __forceinline__ T dfma(T a, T b, T c);
+template <> __device__ __forceinline__ float dfma<float>(float a, float b, float c) { return fmaf(a, b, c); }
+template <> __device__ __forceinline__ double dfma<double>(double a, double b, double c) { return fma(a, b, c); }
+
+// Position p (interior coordinates, -2..n+1 for ghosts) -> interior source
+// cell and whether the normal velocity flips (wall).  Positions further out
+// only feed cells that are never stored; they are clamped.
+struct AxisMap { int idx; int flip; };
+__device__ __forceinline__ AxisMap map_axis(int p, int n, int lo, int hi)
+{
+    if (p < 0) {
+        if (p < -2) return {0, 0};
+        if (lo == BC_PERIODIC) return {p + n, 0};
+        if (lo == BC_OUTFLOW) return {0, 0};
+        return {-1 - p, 1};           // wall: -1 -> 0, -2 -> 1
+    }
+    if (p >= n) {
+        if (p > n + 1) return {n - 1, 0};
+        if (hi == BC_PERIODIC) return {p - n, 0};
+        if (hi == BC_OUTFLOW) return {n - 1, 0};
+        return {2 * n - 1 - p, 1};    // wall: n -> n-1, n+1 -> n-2
+    }
+    return {p, 0};
+}
+
+// Tile-local position whose state a tile cell at local p takes (see the
+// header): periodic keeps p (the LDS ring holds wrapped primitives), outflow
+// and wall use the adjacent interior cell.  Result stays in [-1, tn].
+__device__ __forceinline__ int eval_local(int p, int o, int n, int lo, int hi, int tn)
+{
+    int g = o + p;
+    if (g < 0) { if (lo != BC_PERIODIC) g = 0; }
+    else if (g >= n) { if (hi != BC_PERIODIC) g = n - 1; }
+    int q = g - o;
+    return q < -1 ? -1 : (q > tn ? tn : q);
+}
+
+__device__ __forceinline__ bool wall_ghost(int p, int o, int n, int lo, int hi)
+{
+    int g = o + p;
+    return (g == -1 && lo == BC_WALL) || (g == n && hi == BC_WALL);
+}
+
+// calc_primitives (physics.rs:109-141) for one cell.
+template <typename T, bool FAST>
+__device__ __forceinline__ void to_prim(const T u[5], T V, T invV, T gm1, T w[5])
+{
+    if constexpr (!FAST) {
+        const T r = u[0] / V;
+        const T ux = u[1] / (r * V), uy = u[2] / (r * V), uz = u[3] / (r * V);
+        w[0] = r; w[1] = ux; w[2] = uy; w[3] = uz;
+        w[4] = (u[4] / V - T(0.5) * r * (ux * ux + uy * uy + uz * uz)) * gm1;
+    } else {
+        const T r = u[0] * invV;
+        const T im = drcp<T>(u[0]);
+        const T ux = u[1] * im, uy = u[2] * im, uz = u[3] * im;
+        w[0] = r; w[1] = ux; w[2] = uy; w[3] = uz;
+        const T v2 = dfma<T>(uz, uz, dfma<T>(uy, uy, ux * ux));
+        w[4] = dfma<T>(u[4], invV, -T(0.5) * r * v2) * gm1;
+    }
+}
+
+// Wave-speed denominator of calc_timestep: sqrt(gamma p / rho) + |v|.
+template <typename T, bool FAST>
+__device__ __forceinline__ T wave_den(const T w[5], T gamma)
+{
+    if constexpr (!FAST) {
+        return dsqrt<T>(gamma * w[4] / w[0]) + dsqrt<T>(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
+    } else {
+        const T v2 = dfma<T>(w[3], w[3], dfma<T>(w[2], w[2], w[1] * w[1]));
+        return dsqrt<T>(gamma * w[4] * drcp<T>(w[0])) + dsqrt<T>(v2);
+    }
+}
+
+// Limited slope (extension; the reference's limiters are dead code,
+// physics.f90:205-238): textbook minmod and van Leer.
+template <typename T>
+__device__ __forceinline__ T limited(T a, T b, int lim)
+{
+    if (!(a * b > T(0))) return T(0);
+    if (lim == LIM_MINMOD) return dabs<T>(a) <= dabs<T>(b) ? a : b;
+    return T(2) * a * b / (a + b);
+}
+
+// Rusanov flux of calc_flux_reconstruction (physics.rs:338-405) for one face.
+// l = state of the cell on the + side ("l" in the reference), r = state of the
+// cell on the - side, both (rho, v_normal, v_t1, v_t2, p); F returns the
+// rotated fluxes multiplied by ds (FAST: not multiplied by ds).
+template <typename T, bool FAST>
+__device__ __forceinline__ void rusanov(const T l[5], const T r[5], T F[5], T gamma, T gm1, T ds,
+                                        bool grav, T G, T h)
+{
+    const T rl = l[0], vxl = l[1], vyl = l[2], vzl = l[3], pl = l[4];
+    const T rr = r[0], vxr = r[1], vyr = r[2], vzr = r[3], pr = r[4];
+    if constexpr (!FAST) {
+        const T en_left = T(0.5) * (rl * (vxl * vxl + vyl * vyl + vzl * vzl)) + (pl / gm1);
+        const T en_right = T(0.5) * (rr * (vxr * vxr + vyr * vyr + vzr * vzr)) + (pr / gm1);
+        const T rho_star = (rl + rr) / T(2.0);
+        const T mx = (rl * vxl + rr * vxr) / T(2.0);
+        const T my = (rl * vyl + rr * vyr) / T(2.0);
+        const T mz = (rl * vzl + rr * vzr) / T(2.0);
+        const T en_star = T(0.5) * (en_left + en_right);
+        const T p_star = gm1 * (en_star - T(0.5) * (mx * mx + my * my + mz * mz) / rho_star);
+        T f1 = (mx * mx / rho_star) + p_star;
+        T f2 = mx * my / rho_star;
+        T f3 = mx * mz / rho_star;
+        T f4 = (en_star + p_star) * (mx / rho_star);
+        if (grav) {
+            f1 += T(1.0) * rho_star * G * h;
+            f4 += T(0.5) * rho_star * (vxl + vxr) * G * h;
+        }
+        const T c_l = dsqrt<T>(gamma * pl / rl) + dabs<T>(vxl);
+        const T c_r = dsqrt<T>(gamma * pr / rr) + dabs<T>(vxr);
+        const T c = dmax<T>(c_l, c_r);
+        T f0 = mx - T(0.5) * c * (rl - rr);
+        f1 -= c * (rl * vxl - rr * vxr) / T(2.0);
+        f2 -= c * (rl * vyl - rr * vyr) / T(2.0);
+        f3 -= c * (rl * vzl - rr * vzr) / T(2.0);
+        f4 -= c * (en_left - en_right) / T(2.0);
+        F[0] = f0 * ds; F[1] = f1 * ds; F[2] = f2 * ds; F[3] = f3 * ds; F[4] = f4 * ds;
+    } else {
+        // sums S = 2 * (star state); fluxes returned as 2 * flux / 1 (the
+        // caller folds the factor 1/2 and ds into the update).
+        const T mxl = rl * vxl, myl = rl * vyl, mzl = rl * vzl;
+        const T mxr = rr * vxr, myr = rr * vyr, mzr = rr * vzr;
+        const T igm1 = T(1) / gm1;
+        const T el = dfma<T>(pl, igm1, T(0.5) * rl * dfma<T>(vzl, vzl, dfma<T>(vyl, vyl, vxl * vxl)));
+        const T er = dfma<T>(pr, igm1, T(0.5) * rr * dfma<T>(vzr, vzr, dfma<T>(vyr, vyr, vxr * vxr)));
+        const T Sr = rl + rr, Sx = mxl + mxr, Sy = myl + myr, Sz = mzl + mzr, Se = el + er;
+        const T iS = drcp<T>(Sr);
+        const T m2 = dfma<T>(Sz, Sz, dfma<T>(Sy, Sy, Sx * Sx));
+        const T P2 = gm1 * dfma<T>(T(-0.5) * m2, iS, Se);   // 2 p*
+        const T un = Sx * iS;                                // u* normal
+        T f0 = Sx;
+        T f1 = dfma<T>(Sx, un, P2);
+        T f2 = Sy * un;
+        T f3 = Sz * un;
+        T f4 = (Se + P2) * un;
+        if (grav) {
+            const T gh = G * h;
+            f1 = dfma<T>(Sr, gh, f1);
+            f4 = dfma<T>(T(0.5) * Sr * (vxl + vxr), gh, f4);
+        }
+        const T al = dsqrt<T>(gamma * pl * drcp<T>(rl)) + dabs<T>(vxl);
+        const T ar = dsqrt<T>(gamma * pr * drcp<T>(rr)) + dabs<T>(vxr);
+        const T c = dmax<T>(al, ar);
+        F[0] = dfma<T>(-c, rl - rr, f0);
+        F[1] = dfma<T>(-c, mxl - mxr, f1);
+        F[2] = dfma<T>(-c, myl - myr, f2);
+        F[3] = dfma<T>(-c, mzl - mzr, f3);
+        F[4] = dfma<T>(-c, el - er, f4);
+        (void)ds;
+    }
+}
+
+// Rotated slots per axis (physics.rs:616-677): field index of slot s.
+__device__ __forceinline__ int rot_field(int axis, int slot)
+{
+    // x: (rho, vx, vy, vz, p); y: (rho, vy, vx, vz, p); z: (rho, vz, vx, vy, p)
+    if (slot == 0 || slot == 4 || axis == 0) return slot;
+    if (axis == 1) return slot == 1 ? 2 : (slot == 2 ? 1 : 3);
+    return slot == 1 ? 3 : (slot == 2 ? 1 : 2);
+}
+
+template <typename T>
+struct Consts {
+    T ds, gamma, gm1, V, invV, two_ds, inv_two_ds, half_ds, half_dt, dtds, G;
+    int lim;
+};
+
+// LDS layout of the step kernel.
+template <typename T, int TY>
+struct StepLDS {
+    T W[3][5][TY + 4][RZ];   // primitives ring: planes t-1, t, t+1 (by x mod 3)
+    T Ym[5][TY + 1][TZ];     // -y face state of rows 0..TY
+    T Yb[5][TZ];             // flux through the tile's lower y face
+    T FY[5][TY][TZ];         // flux through face (row, row+1)
+    T Zp[5][TY];             // +z face state of column -1
+    T Zm[5][TY];             // -z face state of column 64
+    T Z0[5][TY];             // -z face state of column 0
+    T FZb[5][TY];            // flux through the tile's lower z face
+};
+
+// Slopes and Hancock-extrapolated state of the cell at LDS region position
+// (ry, rz) of plane slot s0 (neighbours in slots sm, sp).  g[f*3+d] holds the
+// slope (gradient) of field f along axis d.
+template <typename T, int TY, bool FAST>
+__device__ __forceinline__ void cell_xtr(const StepLDS<T, TY> &L, int sm, int s0, int sp, int ry, int rz,
+                                         const Consts<T> &C, T xtr[5], T g[15])
+{
+    T w[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        const T c = L.W[s0][f][ry][rz];
+        const T xm = L.W[sm][f][ry][rz], xp = L.W[sp][f][ry][rz];
+        const T ym = L.W[s0][f][ry - 1][rz], yp = L.W[s0][f][ry + 1][rz];
+        const T zm = L.W[s0][f][ry][rz - 1], zp = L.W[s0][f][ry][rz + 1];
+        w[f] = c;
+        if (C.lim == LIM_NONE) {
+            if constexpr (!FAST) {
+                g[f * 3 + 0] = (xp - xm) / C.two_ds;
+                g[f * 3 + 1] = (yp - ym) / C.two_ds;
+                g[f * 3 + 2] = (zp - zm) / C.two_ds;
+            } else {
+                g[f * 3 + 0] = (xp - xm) * C.inv_two_ds;
+                g[f * 3 + 1] = (yp - ym) * C.inv_two_ds;
+                g[f * 3 + 2] = (zp - zm) * C.inv_two_ds;
+            }
+        } else {
+            g[f * 3 + 0] = limited<T>(c - xm, xp - c, C.lim) / C.ds;
+            g[f * 3 + 1] = limited<T>(c - ym, yp - c, C.lim) / C.ds;
+            g[f * 3 + 2] = limited<T>(c - zm, zp - c, C.lim) / C.ds;
+        }
+    }
+    const T r = w[0], ux = w[1], uy = w[2], uz = w[3], p = w[4];
+    const T drx = g[0], dry = g[1], drz = g[2];
+    const T dxx = g[3], dxy = g[4], dxz = g[5];
+    const T dyx = g[6], dyy = g[7], dyz = g[8];
+    const T dzx = g[9], dzy = g[10], dzz = g[11];
+    const T dpx = g[12], dpy = g[13], dpz = g[14];
+    const T hd = C.half_dt;
+    if constexpr (!FAST) {
+        // extrapolate_primitives_t (physics.rs:251-278), same association
+        xtr[0] = r - hd * (ux * drx + r * dxx + uy * dry + r * dyy + uz * drz + r * dzz);
+        xtr[1] = ux - hd * (ux * dxx + uy * dxy + uz * dxz + (T(1.0) / r) * dpx);
+        xtr[2] = uy - hd * (ux * dyx + uy * dyy + uz * dyz + (T(1.0) / r) * dpy);
+        xtr[3] = uz - hd * (ux * dzx + uy * dzy + uz * dzz + (T(1.0) / r) * dpz);
+        xtr[4] = p - hd * (C.gamma * p * (dxx + dyy + dzz) + ux * dpx + uy * dpy + uz * dpz);
+    } else {
+        const T ir = drcp<T>(r);
+        const T div = dxx + dyy + dzz;
+        xtr[0] = dfma<T>(-hd, dfma<T>(uz, drz, dfma<T>(uy, dry, dfma<T>(ux, drx, r * div))), r);
+        xtr[1] = dfma<T>(-hd, dfma<T>(ir, dpx, dfma<T>(uz, dxz, dfma<T>(uy, dxy, ux * dxx))), ux);
+        xtr[2] = dfma<T>(-hd, dfma<T>(ir, dpy, dfma<T>(uz, dyz, dfma<T>(uy, dyy, ux * dyx))), uy);
+        xtr[3] = dfma<T>(-hd, dfma<T>(ir, dpz, dfma<T>(uz, dzz, dfma<T>(uy, dzy, ux * dzx))), uz);
+        xtr[4] = dfma<T>(-hd, dfma<T>(uz, dpz, dfma<T>(uy, dpy, dfma<T>(ux, dpx, C.gamma * p * div))), p);
+    }
+}
+
+// Normal slope along axis d at region position (ry, rz) of slot s0 (used for
+// a wall ghost, whose primitives in the ring are the mirrored ones).
+template <typename T, int TY>
+__device__ __forceinline__ T slope_at(const StepLDS<T, TY> &L, int sm, int s0, int sp, int ry, int rz,
+                                      int f, int d, const Consts<T> &C)
+{
+    const T c = L.W[s0][f][ry][rz];
+    T m, p;
+    if (d == 0) { m = L.W[sm][f][ry][rz]; p = L.W[sp][f][ry][rz]; }
+    else if (d == 1) { m = L.W[s0][f][ry - 1][rz]; p = L.W[s0][f][ry + 1][rz]; }
+    else { m = L.W[s0][f][ry][rz - 1]; p = L.W[s0][f][ry][rz + 1]; }
+    if (C.lim == LIM_NONE) return (p - m) / C.two_ds;
+    return limited<T>(c - m, p - c, C.lim) / C.ds;
+}
+
+// Face state of a cell along axis d: xtr + sign * slope * ds/2 (physics.rs:313-336).
+template <typename T, bool FAST>
+__device__ __forceinline__ void face_state(const T xtr[5], const T g[15], int d, bool plus, const Consts<T> &C,
+                                           T s[5])
+{
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        const T q = g[f * 3 + d] * C.half_ds;
+        s[f] = plus ? xtr[f] + q : xtr[f] - q;
+    }
+}
+
+// Evaluate the state of tile cell (py, pz) of the current plane per the
+// boundary rules, returning its extrapolated state and slopes.
+template <typename T, int TY, bool FAST>
+__device__ __forceinline__ void eval_cell(const StepLDS<T, TY> &L, int sm, int s0, int sp, int py, int pz,
+                                          int y0, int z0, const StepArgs &A, const Consts<T> &C, T xtr[5],
+                                          T g[15])
+{
+    const int qy = eval_local(py, y0, A.n[1], A.bc[2], A.bc[3], TY);
+    const int qz = eval_local(pz, z0, A.n[2], A.bc[4], A.bc[5], TZ);
+    cell_xtr<T, TY, FAST>(L, sm, s0, sp, qy + 2, qz + 2, C, xtr, g);
+    if (wall_ghost(py, y0, A.n[1], A.bc[2], A.bc[3])) {
+        xtr[2] = -xtr[2];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) g[f * 3 + 1] = slope_at<T, TY>(L, sm, s0, sp, py + 2, qz + 2, f, 1, C);
+    }
+    if (wall_ghost(pz, z0, A.n[2], A.bc[4], A.bc[5])) {
+        xtr[3] = -xtr[3];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) g[f * 3 + 2] = slope_at<T, TY>(L, sm, s0, sp, qy + 2, pz + 2, f, 2, C);
+    }
+}
+
+// Flux through a face along `axis` from unrotated states (l on the + side).
+template <typename T, bool FAST>
+__device__ __forceinline__ void face_flux(const T l[5], const T r[5], int axis, const Consts<T> &C, bool grav,
+                                          T h, T out[5])
+{
+    T lr[5], rr[5], F[5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) { lr[s] = l[rot_field(axis, s)]; rr[s] = r[rot_field(axis, s)]; }
+    rusanov<T, FAST>(lr, rr, F, C.gamma, C.gm1, C.ds, grav, C.G, h);
+#pragma unroll
+    for (int s = 0; s < 5; ++s) out[rot_field(axis, s)] = F[s];
+}
+
+// Gravity height of physics.rs:368 for the cell with interior z index z.
+template <typename T>
+__device__ __forceinline__ T grav_h(int z, int nz, T ds)
+{
+    const int kg = z + 2, nzg = nz + 4;
+    return (T(nzg) - T(2.0) * T(2)) * ds - (T(kg) - T(2.0)) * ds;
+}
+
+__device__ __forceinline__ int ring(int x) { return (x + 3) % 3; }  // x >= -3
+
+// Load plane xg's primitives (tile plus two-cell halo) into ring slot `slot`.
+// The thread's own cell's conserved values are returned in u_own.
+template <typename T, int TY, bool FAST>
+__device__ __forceinline__ void load_plane(StepLDS<T, TY> &L, int slot, int xg, int y0, int z0, const StepArgs &A,
+                                           const Fields<const T> &U, const Consts<T> &C, int tid)
+{
+    constexpr int NT = TY * TZ;
+    constexpr int NHALO = 4 * RZ + 4 * TY;
+    const AxisMap mx = map_axis(xg, A.n[0], A.bc[0], A.bc[1]);
+    const size_t plane = (size_t)mx.idx * A.n[1] * A.n[2];
+    for (int c = tid; c < NT + NHALO; c += NT) {
+        int ry, rz;
+        if (c < NT) { ry = (c >> 6) + 2; rz = (c & 63) + 2; }
+        else {
+            const int h = c - NT;
+            if (h < 4 * RZ) {
+                const int rr = h / RZ;
+                ry = rr < 2 ? rr : TY + rr;   // rows 0,1, TY+2, TY+3
+                rz = h % RZ;
+            } else {
+                const int hh = h - 4 * RZ;
+                ry = 2 + (hh >> 2);
+                const int cc = hh & 3;
+                rz = cc < 2 ? cc : TZ + cc;   // cols 0,1, 66,67
+            }
+        }
+        const AxisMap my = map_axis(y0 + ry - 2, A.n[1], A.bc[2], A.bc[3]);
+        const AxisMap mz = map_axis(z0 + rz - 2, A.n[2], A.bc[4], A.bc[5]);
+        const size_t idx = plane + (size_t)my.idx * A.n[2] + mz.idx;
+        T u[5], w[5];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) u[f] = U.f[f][idx];
+        to_prim<T, FAST>(u, C.V, C.invV, C.gm1, w);
+        if (mx.flip) w[1] = -w[1];
+        if (my.flip) w[2] = -w[2];
+        if (mz.flip) w[3] = -w[3];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) L.W[slot][f][ry][rz] = w[f];
+    }
+}
+
+// Deterministic block reduction of (max den, nan flag, 5 sums) into partials.
+template <int NT>
+__device__ __forceinline__ void block_partials(double den, int nanf, const double sums[5], double *red,
+                                               double *partials, int blk, int tid)
+{
+    double v[7] = {den, (double)nanf, sums[0], sums[1], sums[2], sums[3], sums[4]};
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        v[0] = fmax(v[0], __shfl_xor(v[0], off));
+        v[1] = fmax(v[1], __shfl_xor(v[1], off));
+#pragma unroll
+        for (int q = 2; q < 7; ++q) v[q] += __shfl_xor(v[q], off);
+    }
+    constexpr int NW = NT / 64;
+    const int w = tid >> 6, lane = tid & 63;
+    if (lane == 0)
+        for (int q = 0; q < 7; ++q) red[w * 8 + q] = v[q];
+    __syncthreads();
+    if (tid == 0) {
+        double o[7];
+        for (int q = 0; q < 7; ++q) o[q] = red[q];
+        for (int ww = 1; ww < NW; ++ww) {
+            o[0] = fmax(o[0], red[ww * 8 + 0]);
+            o[1] = fmax(o[1], red[ww * 8 + 1]);
+            for (int q = 2; q < 7; ++q) o[q] += red[ww * 8 + q];
+        }
+        double *P = partials + (size_t)blk * PARTIAL_STRIDE;
+        for (int q = 0; q < 7; ++q) P[q] = o[q];
+        P[7] = 0.0;
+    }
+}
+
+template <typename T, int TY, bool FAST, bool BULK>
+__global__ void __launch_bounds__(TY * TZ)
+step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, const Scalars *__restrict__ sc,
+            double *__restrict__ partials)
+{
+    constexpr int NT = TY * TZ;
+    __shared__ StepLDS<T, TY> L;
+    __shared__ double red[(NT / 64) * 8];
+
+    const int tid = threadIdx.x;
+    const int ty = tid >> 6, tz = tid & 63;
+    const int nx = A.n[0], ny = A.n[1], nz = A.n[2];
+#ifdef DG_EMU_POISON
+    {   // CPU-emulation builds only: make every unwritten LDS read visible
+        T *raw = reinterpret_cast<T *>(&L);
+        for (size_t q = tid; q < sizeof(L) / sizeof(T); q += NT) raw[q] = T(NAN);
+        __syncthreads();
+    }
+#endif
+
+    // tile of this workgroup (optionally spread so each XCD gets a
+    // contiguous range of neighbouring tiles)
+    int b = blockIdx.x;
+    if (A.swizzle) b = (b & 7) * (A.nblocks >> 3) + (b >> 3);
+    const int bz = b % A.nbz;
+    const int by = (b / A.nbz) % A.nby;
+    const int bx = b / (A.nbz * A.nby);
+    const int x0 = bx * A.lx, y0 = by * TY, z0 = bz * TZ;
+    const int xend = min(x0 + A.lx, nx);
+    const int yc = y0 + ty, zc = z0 + tz;
+    const bool valid = yc < ny && zc < nz;
+    const size_t plane_sz = (size_t)ny * nz;
+    const size_t col = (size_t)yc * nz + zc;
+
+    Consts<T> C;
+    C.ds = T(A.ds);
+    C.gamma = T(A.gamma);
+    C.gm1 = C.gamma - T(1.0);
+    C.V = C.ds * C.ds * C.ds;
+    C.invV = T(1) / C.V;
+    C.two_ds = T(2.0) * C.ds;
+    C.inv_two_ds = T(1) / C.two_ds;
+    C.half_ds = C.ds / T(2.);
+    C.G = T(A.g);
+    C.lim = A.limiter;
+    const T dt = T(sc->dt);
+    C.half_dt = T(0.5) * dt;
+    C.dtds = FAST ? dt * C.ds * C.ds * T(0.5) : dt * C.ds;
+
+    double sums[5] = {0, 0, 0, 0, 0};
+    double dmaxden = 0.0;
+    T maxden = T(0);
+    int nanf = 0;
+
+    if (!sc->active) {
+        // inactive step of a captured run: carry the state through unchanged
+        if (valid)
+            for (int x = x0; x < xend; ++x)
+                for (int f = 0; f < 5; ++f) Vout.f[f][x * plane_sz + col] = U.f[f][x * plane_sz + col];
+        return;
+    }
+
+    // per-thread carried values (own column)
+    T xprev[5], gxprev[5];           // xtr and x-slope of plane t-1
+    T fxlo[5], fyu[5], fyl[5], fzu[5], fzl[5];  // flux parts of plane t-1's cell
+#pragma unroll
+    for (int f = 0; f < 5; ++f) { xprev[f] = gxprev[f] = fxlo[f] = fyu[f] = fyl[f] = fzu[f] = fzl[f] = T(0); }
+
+    load_plane<T, TY, FAST>(L, ring(x0 - 2), x0 - 2, y0, z0, A, U, C, tid);
+    load_plane<T, TY, FAST>(L, ring(x0 - 1), x0 - 1, y0, z0, A, U, C, tid);
+
+    const bool zgrav = A.gravity != 0;
+    for (int t = x0 - 1; t <= xend; ++t) {
+        const bool full = t >= x0 && t < xend;
+        const int sm = ring(t - 1), s0 = ring(t), sp = ring(t + 1);
+        // ---- A: primitives of plane t+1 into the ring
+        load_plane<T, TY, FAST>(L, sp, t + 1, y0, z0, A, U, C, tid);
+        __syncthreads();
+
+        // ---- B: extrapolated state and slopes of the own cell
+        T xtr[5], g[15];
+        eval_cell<T, TY, FAST>(L, sm, s0, sp, ty, tz, y0, z0, A, C, xtr, g);
+        T szm[5];
+        face_state<T, FAST>(xtr, g, 2, false, C, szm);
+        if (full) {
+            T sym[5];
+            face_state<T, FAST>(xtr, g, 1, false, C, sym);
+#pragma unroll
+            for (int f = 0; f < 5; ++f) L.Ym[f][ty][tz] = sym[f];
+            if (tz == 0) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) L.Z0[f][ty] = szm[f];
+            }
+            // halo states
+            if (ty == 0) {
+                T hx[5], hg[15], hs[5], F[5];
+                eval_cell<T, TY, FAST>(L, sm, s0, sp, -1, tz, y0, z0, A, C, hx, hg);
+                face_state<T, FAST>(hx, hg, 1, true, C, hs);
+                face_flux<T, FAST>(sym, hs, 1, C, false, T(0), F);   // face (-1, 0)
+#pragma unroll
+                for (int f = 0; f < 5; ++f) L.Yb[f][tz] = F[f];
+            } else if (ty == 1) {
+                T hx[5], hg[15], hs[5];
+                eval_cell<T, TY, FAST>(L, sm, s0, sp, TY, tz, y0, z0, A, C, hx, hg);
+                face_state<T, FAST>(hx, hg, 1, false, C, hs);
+#pragma unroll
+                for (int f = 0; f < 5; ++f) L.Ym[f][TY][tz] = hs[f];
+            } else if (ty == 2 && tz < 2 * TY) {
+                const bool lowz = tz < TY;
+                const int row = lowz ? tz : tz - TY;
+                T hx[5], hg[15], hs[5];
+                eval_cell<T, TY, FAST>(L, sm, s0, sp, row, lowz ? -1 : TZ, y0, z0, A, C, hx, hg);
+                face_state<T, FAST>(hx, hg, 2, lowz, C, hs);
+                if (lowz) {
+#pragma unroll
+                    for (int f = 0; f < 5; ++f) L.Zp[f][row] = hs[f];
+                } else {
+#pragma unroll
+                    for (int f = 0; f < 5; ++f) L.Zm[f][row] = hs[f];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- C: fluxes through the own cell's faces
+        T fxu[5] = {T(0), T(0), T(0), T(0), T(0)};  // face (t-1, t) along x
+        if (t >= x0) {
+            T r[5], l[5];
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                const T qp = gxprev[f] * C.half_ds, qc = g[f * 3 + 0] * C.half_ds;
+                r[f] = xprev[f] + qp;
+                l[f] = xtr[f] - qc;
+            }
+            if (t == 0 && A.bc[0] == BC_WALL) {      // ghost plane -1 mirrors plane 0
+#pragma unroll
+                for (int f = 0; f < 5; ++f) r[f] = (f == 1 ? -xtr[f] : xtr[f]) + gxprev[f] * C.half_ds;
+            }
+            if (t == nx && A.bc[1] == BC_OUTFLOW) {  // ghost nx copies plane nx-1
+#pragma unroll
+                for (int f = 0; f < 5; ++f) l[f] = xprev[f] - gxprev[f] * C.half_ds;
+            }
+            if (t == nx && A.bc[1] == BC_WALL) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) l[f] = (f == 1 ? -xprev[f] : xprev[f]) - g[f * 3 + 0] * C.half_ds;
+            }
+            face_flux<T, FAST>(l, r, 0, C, false, T(0), fxu);
+        }
+        T fyu_n[5], fzu_n[5], fyl_n[5], fzl_n[5];
+        if (full) {
+            T r[5], l[5];
+            face_state<T, FAST>(xtr, g, 1, true, C, r);
+#pragma unroll
+            for (int f = 0; f < 5; ++f) l[f] = L.Ym[f][ty + 1][tz];
+            face_flux<T, FAST>(l, r, 1, C, false, T(0), fyu_n);
+#pragma unroll
+            for (int f = 0; f < 5; ++f) L.FY[f][ty][tz] = fyu_n[f];
+            face_state<T, FAST>(xtr, g, 2, true, C, r);
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                const T nb = __shfl_down(szm[f], 1);
+                l[f] = tz == TZ - 1 ? L.Zm[f][ty] : nb;
+            }
+            const T h = zgrav ? grav_h<T>(zc, nz, C.ds) : T(0);
+            face_flux<T, FAST>(l, r, 2, C, zgrav, h, fzu_n);
+            if (ty == 2 && tz < TY) {                // the tile's lower z faces
+                T rb[5], lb[5], Fb[5];
+#pragma unroll
+                for (int f = 0; f < 5; ++f) { rb[f] = L.Zp[f][tz]; lb[f] = L.Z0[f][tz]; }
+                const T hb = zgrav ? grav_h<T>(z0 - 1, nz, C.ds) : T(0);
+                face_flux<T, FAST>(lb, rb, 2, C, zgrav, hb, Fb);
+#pragma unroll
+                for (int f = 0; f < 5; ++f) L.FZb[f][tz] = Fb[f];
+            }
+        }
+        if (full) __syncthreads();
+
+        // ---- D: gather lower y/z fluxes, finish cell t-1, carry plane t
+        if (full) {
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                fyl_n[f] = ty == 0 ? L.Yb[f][tz] : L.FY[f][ty - 1][tz];
+                const T up = __shfl_up(fzu_n[f], 1);
+                fzl_n[f] = tz == 0 ? L.FZb[f][ty] : up;
+            }
+            if (yc == 0 && A.bc[2] == BC_OUTFLOW) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) fyl_n[f] = fyu_n[f];
+            }
+            if (zc == 0 && A.bc[4] == BC_OUTFLOW) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) fzl_n[f] = fzu_n[f];
+            }
+        }
+        const int xc = t - 1;
+        if (xc >= x0 && xc < xend && valid) {
+            if (xc == 0 && A.bc[0] == BC_OUTFLOW) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) fxlo[f] = fxu[f];
+            }
+            const size_t idx = (size_t)xc * plane_sz + col;
+            T un[5];
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                const T u = U.f[f][idx];
+                T acc;
+                if constexpr (!FAST) acc = ((((fxu[f] - fxlo[f]) + fyu[f]) - fyl[f]) + fzu[f]) - fzl[f];
+                else acc = (fxu[f] - fxlo[f]) + fyu[f];   // fyu holds the y/z partial sum
+                un[f] = u - C.dtds * acc;
+                Vout.f[f][idx] = un[f];
+                sums[f] += (double)un[f];
+            }
+            T w[5];
+            to_prim<T, FAST>(un, C.V, C.invV, C.gm1, w);
+            if constexpr (BULK) {
+                double wd[5];
+                for (int f = 0; f < 5; ++f) wd[f] = (double)w[f];
+                const double den = wave_den<double, false>(wd, (double)C.gamma);
+                if (den != den) nanf = 1; else dmaxden = fmax(dmaxden, den);
+            } else {
+                const T den = wave_den<T, FAST>(w, C.gamma);
+                if (den != den) nanf = 1; else maxden = dmax<T>(maxden, den);
+            }
+        }
+        if (full) {
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                fxlo[f] = fxu[f];
+                if constexpr (!FAST) {
+                    fyu[f] = fyu_n[f]; fyl[f] = fyl_n[f]; fzu[f] = fzu_n[f]; fzl[f] = fzl_n[f];
+                } else {
+                    fyu[f] = ((fyu_n[f] - fyl_n[f]) + fzu_n[f]) - fzl_n[f];
+                }
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < 5; ++f) { xprev[f] = xtr[f]; gxprev[f] = g[f * 3 + 0]; }
+        if (!full) __syncthreads();   // ring slot sm is overwritten next iteration
+    }
+
+    if constexpr (!BULK) dmaxden = (double)maxden;
+    block_partials<NT>(dmaxden, nanf, sums, red, partials, blockIdx.x, tid);
+}
+
+// --------------------------------------------------------- small kernels --
+template <typename T, bool BULK>
+__global__ void __launch_bounds__(256)
+reduce_kernel(const StepArgs A, const Fields<const T> U, double *__restrict__ partials)
+{
+    __shared__ double red[4 * 8];
+    const T ds = T(A.ds), gamma = T(A.gamma), gm1 = gamma - T(1.0);
+    const T V = ds * ds * ds;
+    const size_t n3 = (size_t)A.n[0] * A.n[1] * A.n[2];
+    double sums[5] = {0, 0, 0, 0, 0};
+    double dmaxden = 0.0;
+    T maxden = T(0);
+    int nanf = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
+        T u[5], w[5];
+        for (int f = 0; f < 5; ++f) { u[f] = U.f[f][i]; sums[f] += (double)u[f]; }
+        to_prim<T, false>(u, V, T(1) / V, gm1, w);
+        if constexpr (BULK) {
+            double wd[5];
+            for (int f = 0; f < 5; ++f) wd[f] = (double)w[f];
+            const double den = wave_den<double, false>(wd, (double)gamma);
+            if (den != den) nanf = 1; else dmaxden = fmax(dmaxden, den);
+        } else {
+            const T den = wave_den<T, false>(w, gamma);
+            if (den != den) nanf = 1; else maxden = dmax<T>(maxden, den);
+        }
+    }
+    if constexpr (!BULK) dmaxden = (double)maxden;
+    block_partials<256>(dmaxden, nanf, sums, red, partials, blockIdx.x, threadIdx.x);
+}
+
+// dt from the partials (calc_timestep, physics.rs:177-196: CFL * min over
+// cells of ds / den == CFL * ds / max den), conservation sums, and the run
+// clock (main.rs:296-298).  One workgroup; fixed reduction order.
+template <typename T, bool BULK>
+__global__ void __launch_bounds__(1024)
+finalize_kernel(const StepArgs A, Scalars *__restrict__ sc, const double *__restrict__ partials, int np,
+                double cfl, int nan_fallback, int advance)
+{
+    __shared__ double red[16 * 8];
+    const int tid = threadIdx.x;
+    double den = 0.0, nanf = 0.0, s[5] = {0, 0, 0, 0, 0};
+    for (int i = tid; i < np; i += 1024) {
+        const double *P = partials + (size_t)i * PARTIAL_STRIDE;
+        den = fmax(den, P[0]);
+        nanf = fmax(nanf, P[1]);
+        for (int q = 0; q < 5; ++q) s[q] += P[2 + q];
+    }
+    double v[7] = {den, nanf, s[0], s[1], s[2], s[3], s[4]};
+    for (int off = 32; off >= 1; off >>= 1) {
+        v[0] = fmax(v[0], __shfl_xor(v[0], off));
+        v[1] = fmax(v[1], __shfl_xor(v[1], off));
+        for (int q = 2; q < 7; ++q) v[q] += __shfl_xor(v[q], off);
+    }
+    if ((tid & 63) == 0)
+        for (int q = 0; q < 7; ++q) red[(tid >> 6) * 8 + q] = v[q];
+    __syncthreads();
+    if (tid != 0) return;
+    double o[7];
+    for (int q = 0; q < 7; ++q) o[q] = red[q];
+    for (int w = 1; w < 16; ++w) {
+        o[0] = fmax(o[0], red[w * 8 + 0]);
+        o[1] = fmax(o[1], red[w * 8 + 1]);
+        for (int q = 2; q < 7; ++q) o[q] += red[w * 8 + q];
+    }
+    if (advance) {
+        if (!sc->active) return;
+        if constexpr (!BULK) sc->t = (double)(T(sc->t) + T(sc->dt));
+        else sc->t = sc->t + sc->dt;
+        sc->steps += 1;
+    }
+    const bool nan = o[1] != 0.0;
+    double dt;
+    if constexpr (!BULK) {
+        const T minval = nan ? (nan_fallback ? T(1.0) : T(NAN)) : T(A.ds) / T(o[0]);
+        dt = (double)(T(cfl) * minval);
+    } else {
+        const double minval = nan ? (nan_fallback ? 1.0 : NAN) : A.ds / o[0];
+        dt = cfl * minval;
+    }
+    sc->dt = dt;
+    sc->nonfinite = nan ? 1 : 0;
+    sc->maxden = o[0];
+    for (int q = 0; q < 5; ++q) sc->sums[q] = o[2 + q];
+    sc->active = !((sc->t > sc->t_end) || (sc->steps >= sc->max_steps));
+}
+
+// calc_conserved (physics.rs:143-175), in place on interior arrays holding W.
+template <typename T>
+__global__ void __launch_bounds__(256) prim_to_cons_kernel(const StepArgs A, const Fields<T> U)
+{
+    const T ds = T(A.ds), gm1 = T(A.gamma) - T(1.0);
+    const T V = ds * ds * ds;
+    const size_t n3 = (size_t)A.n[0] * A.n[1] * A.n[2];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
+        const T r = U.f[0][i], vx = U.f[1][i], vy = U.f[2][i], vz = U.f[3][i], p = U.f[4][i];
+        U.f[0][i] = r * V;
+        U.f[1][i] = vx * r * V;
+        U.f[2][i] = vy * r * V;
+        U.f[3][i] = vz * r * V;
+        U.f[4][i] = V * (r * (vx * vx + vy * vy + vz * vz) / T(2.0) + p / gm1);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+cons_to_prim_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> W)
+{
+    const T ds = T(A.ds), gm1 = T(A.gamma) - T(1.0);
+    const T V = ds * ds * ds;
+    const size_t n3 = (size_t)A.n[0] * A.n[1] * A.n[2];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
+        T u[5], w[5];
+        for (int f = 0; f < 5; ++f) u[f] = U.f[f][i];
+        to_prim<T, false>(u, V, T(1) / V, gm1, w);
+        for (int f = 0; f < 5; ++f) W.f[f][i] = w[f];
+    }
+}
+
+// ---------------------------------------------------------------- launch --
+template <typename T> static Fields<const T> cfields(const void *const p[5])
+{
+    Fields<const T> F;
+    for (int f = 0; f < 5; ++f) F.f[f] = static_cast<const T *>(p[f]);
+    return F;
+}
+template <typename T> static Fields<T> mfields(void *const p[5])
+{
+    Fields<T> F;
+    for (int f = 0; f < 5; ++f) F.f[f] = static_cast<T *>(p[f]);
+    return F;
+}
+
+template <typename T, int TY, bool FAST, bool BULK>
+static void launch_step_t(const StepArgs &a, const void *const in[5], void *const out[5], const Scalars *sc,
+                          double *partials, hipStream_t s)
+{
+    hipLaunchKernelGGL((step_kernel<T, TY, FAST, BULK>), dim3(a.nblocks), dim3(TY * TZ), 0, s, a,
+                       cfields<T>(in), mfields<T>(out), sc, partials);
+}
+
+int launch_step(int prec, int fast, int tile_y, const StepArgs &a, const void *const in[5], void *const out[5],
+                const Scalars *sc, double *partials, hipStream_t s)
+{
+#define DG_CASE(T, TYV, F, B) launch_step_t<T, TYV, F, B>(a, in, out, sc, partials, s)
+    if (prec == 0) {
+        if (tile_y == 4) DG_CASE(double, 4, false, false);
+        else if (tile_y == 8) DG_CASE(double, 8, false, false);
+        else return -1;
+    } else {
+        const bool bulk = prec == 2;
+        if (tile_y == 4) {
+            if (fast) { if (bulk) DG_CASE(float, 4, true, true); else DG_CASE(float, 4, true, false); }
+            else { if (bulk) DG_CASE(float, 4, false, true); else DG_CASE(float, 4, false, false); }
+        } else if (tile_y == 8) {
+            if (fast) { if (bulk) DG_CASE(float, 8, true, true); else DG_CASE(float, 8, true, false); }
+            else { if (bulk) DG_CASE(float, 8, false, true); else DG_CASE(float, 8, false, false); }
+        } else if (tile_y == 16) {
+            if (fast) { if (bulk) DG_CASE(float, 16, true, true); else DG_CASE(float, 16, true, false); }
+            else { if (bulk) DG_CASE(float, 16, false, true); else DG_CASE(float, 16, false, false); }
+        } else return -1;
+    }
+#undef DG_CASE
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int step_lds_bytes(int prec, int tile_y)
+{
+    const size_t el = prec == 0 ? sizeof(double) : sizeof(float);
+    const size_t ty = (size_t)tile_y;
+    const size_t cells = 3 * 5 * (ty + 4) * RZ + 5 * (ty + 1) * TZ + 5 * TZ + 5 * ty * TZ + 4 * 5 * ty;
+    return (int)(cells * el + (ty * TZ / 64) * 8 * sizeof(double));
+}
+
+int reduce_blocks(const StepArgs &a)
+{
+    const size_t n3 = (size_t)a.n[0] * a.n[1] * a.n[2];
+    size_t nb = (n3 + 255) / 256;
+    if (nb > 2048) nb = 2048;
+    if (nb < 1) nb = 1;
+    return (int)nb;
+}
+
+int launch_reduce(int prec, const StepArgs &a, const void *const u[5], double *partials, int nblocks,
+                  hipStream_t s)
+{
+    if (prec == 0)
+        hipLaunchKernelGGL((reduce_kernel<double, false>), dim3(nblocks), dim3(256), 0, s, a, cfields<double>(u), partials);
+    else if (prec == 1)
+        hipLaunchKernelGGL((reduce_kernel<float, false>), dim3(nblocks), dim3(256), 0, s, a, cfields<float>(u), partials);
+    else
+        hipLaunchKernelGGL((reduce_kernel<float, true>), dim3(nblocks), dim3(256), 0, s, a, cfields<float>(u), partials);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_finalize(int prec, const StepArgs &a, Scalars *sc, const double *partials, int nparts, double cfl,
+                    int nan_fallback, int advance, hipStream_t s)
+{
+    if (prec == 0)
+        hipLaunchKernelGGL((finalize_kernel<double, false>), dim3(1), dim3(1024), 0, s, a, sc, partials, nparts, cfl, nan_fallback, advance);
+    else if (prec == 1)
+        hipLaunchKernelGGL((finalize_kernel<float, false>), dim3(1), dim3(1024), 0, s, a, sc, partials, nparts, cfl, nan_fallback, advance);
+    else
+        hipLaunchKernelGGL((finalize_kernel<float, true>), dim3(1), dim3(1024), 0, s, a, sc, partials, nparts, cfl, nan_fallback, advance);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_prim_to_cons(int prec, const StepArgs &a, void *const u[5], hipStream_t s)
+{
+    const int nb = reduce_blocks(a);
+    if (prec == 0) hipLaunchKernelGGL((prim_to_cons_kernel<double>), dim3(nb), dim3(256), 0, s, a, mfields<double>(u));
+    else hipLaunchKernelGGL((prim_to_cons_kernel<float>), dim3(nb), dim3(256), 0, s, a, mfields<float>(u));
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_cons_to_prim(int prec, const StepArgs &a, const void *const u[5], void *const w[5], hipStream_t s)
+{
+    const int nb = reduce_blocks(a);
+    if (prec == 0)
+        hipLaunchKernelGGL((cons_to_prim_kernel<double>), dim3(nb), dim3(256), 0, s, a, cfields<double>(u), mfields<double>(w));
+    else
+        hipLaunchKernelGGL((cons_to_prim_kernel<float>), dim3(nb), dim3(256), 0, s, a, cfields<float>(u), mfields<float>(w));
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+} // namespace dg

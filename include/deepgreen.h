@@ -1,0 +1,144 @@
+/*
+ * deepgreen.h -- C ABI of the MI355X (gfx950) compressible-Euler step library.
+ *
+ * This is the drop-in boundary for the reference's hot path: the Rust host
+ * (src/rs/src/main.rs, init.rs, grid.rs) keeps its state layout, its
+ * set-up and its time loop, and replaces
+ *     physics::compute_step(info, grid, &mut dt)      src/rs/src/physics.rs:595
+ * and the loop around it
+ *     compute(info, grid, total_time, max_steps, tout) src/rs/src/main.rs:270-300
+ * by the calls below.  Plain C types only; every call returns 0 or a negative
+ * dg_status and never aborts.  A dg_ctx is used by one host thread at a time.
+ *
+ * Host arrays crossing this boundary use the reference layout: one array per
+ * field, C order over the ghosted grid (n+4 cells per axis, k fastest), the
+ * layout of Array3<f32> in grid.rs:79-109, with conserved values extensive
+ * (multiplied by the cell volume ds^3, physics.rs:143-175).  The element type
+ * is double for DG_FP64 and float for DG_FP32 / DG_FP32_BULK.
+ */
+#ifndef DEEPGREEN_H
+#define DEEPGREEN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dg_status {
+    DG_OK = 0,
+    DG_ERR_INVALID = -1,     /* bad argument or configuration */
+    DG_ERR_DEVICE = -2,      /* HIP runtime error */
+    DG_ERR_ALLOC = -3,       /* device or host allocation failed */
+    DG_ERR_NONFINITE = -4,   /* dt is not finite (only when dt_nan_fallback = 0) */
+    DG_ERR_STATE = -5        /* no state set yet */
+} dg_status;
+
+/* BoundaryType, grid.rs:24-28 (same numbering). */
+typedef enum dg_boundary { DG_PERIODIC = 0, DG_OUTFLOW = 1, DG_WALL = 2 } dg_boundary;
+
+typedef enum dg_precision {
+    DG_FP64 = 0,      /* state, stencil, dt and sums in double */
+    DG_FP32 = 1,      /* everything in float, dt as the reference f32 solver */
+    DG_FP32_BULK = 2  /* float state and stencil; dt reduction and sums in double */
+} dg_precision;
+
+typedef enum dg_math {
+    DG_MATH_EXACT = 0, /* the reference's operation order, IEEE div/sqrt, no FMA */
+    DG_MATH_FAST = 1   /* same scheme, reassociated with FMA and reciprocals */
+} dg_math;
+
+/* Slope used by the MUSCL-Hancock reconstruction.  NONE is the reference's
+ * unlimited central difference (physics.rs:198-215). */
+typedef enum dg_limiter { DG_LIMITER_NONE = 0, DG_LIMITER_MINMOD = 1, DG_LIMITER_VANLEER = 2 } dg_limiter;
+
+typedef enum dg_state_kind { DG_PRIMITIVE = 0, DG_CONSERVED = 1 } dg_state_kind;
+
+/* Replaces Simulation (grid.rs:31-41) plus the compile-time constants of
+ * globals.rs:3-7, which become parameters. */
+typedef struct dg_config {
+    int64_t n[3];          /* interior cells per axis (Simulation::new arguments) */
+    int32_t nghosts;       /* must be 2 (the reference's only value) */
+    int32_t bc[6];         /* x-, x+, y-, y+, z-, z+ ; periodic faces come in pairs */
+    double ds;             /* cell size */
+    double gamma;          /* GAMMA */
+    double cfl;            /* CFL */
+    double g;              /* G, used when gravity != 0 */
+    int32_t gravity;       /* Simulation::gravity: z-flux gravity term, physics.rs:366-372 */
+    int32_t precision;     /* dg_precision */
+    int32_t math;          /* dg_math */
+    int32_t limiter;       /* dg_limiter */
+    int32_t dt_nan_fallback; /* 1: a NaN wave speed gives dt = CFL*1.0 (physics.rs:191-194) */
+    int32_t device;        /* HIP device ordinal */
+    int32_t tile_y;        /* 0 = automatic; else rows per workgroup (4, 8 or 16) */
+    int32_t chunk_x;       /* 0 = automatic; else planes per workgroup along x */
+} dg_config;
+
+typedef struct dg_ctx dg_ctx;
+
+/* Fill *cfg with the reference defaults: gamma 5/3, CFL 0.2, G -0.1, no
+ * gravity, periodic everywhere, FP32 exact, no limiter (globals.rs:3-7). */
+void dg_config_default(dg_config *cfg);
+
+/* Replaces Simulation::new + SimulationGrid::new (grid.rs:44-76, 169-221):
+ * allocates the device state (two conserved buffers) and a HIP stream. */
+int dg_create(const dg_config *cfg, dg_ctx **out);
+void dg_destroy(dg_ctx *ctx);
+const char *dg_last_error(const dg_ctx *ctx);
+
+/* Upload a state from host arrays f[0..4] in the reference layout.
+ * kind = DG_PRIMITIVE: (rho, vx, vy, vz, p); the library then applies
+ * calc_conserved (physics.rs:143-175).  kind = DG_CONSERVED: (mass,
+ * momentum_x, momentum_y, momentum_z, energy).  Only interior cells are read. */
+int dg_set_state(dg_ctx *ctx, const void *const f[5], int kind);
+
+/* Download the current state into host arrays in the reference layout.
+ * Interior cells are written; ghost cells are filled with the boundary maps
+ * (boundaries.rs:5-99, handle_walls :197-407) for both kinds. */
+int dg_get_state(dg_ctx *ctx, void *const f[5], int kind);
+
+/* One call of physics::compute_step (physics.rs:595-681): computes dt from the
+ * current state (returned in *dt_used), advances the state by one step. */
+int dg_step(dg_ctx *ctx, double *dt_used);
+
+/* The loop of main.rs:283-299 without output: steps while
+ * !(t > t_end || steps >= max_steps), accumulating t += dt in the precision's
+ * time type (float for DG_FP32, as the reference).  Runs from a captured HIP
+ * graph with the dt kept on the device.  t and the step count continue from
+ * previous calls; *t_out and *steps_out receive the totals. */
+int dg_run(dg_ctx *ctx, int64_t max_steps, double t_end, double *t_out, int64_t *steps_out);
+
+/* Reset the run clock (t = 0, steps = 0) kept by dg_run. */
+int dg_reset_clock(dg_ctx *ctx);
+
+/* FP64 sums over the interior of the five conserved fields of the current
+ * state (total mass, momentum, energy).  No reference counterpart: the
+ * reference has no conservation diagnostic. */
+int dg_conservation_sums(dg_ctx *ctx, double out[5]);
+
+/* dt the next step will use (CFL * min over cells, physics.rs:177-196). */
+int dg_next_dt(dg_ctx *ctx, double *dt);
+
+/* Device pointers of the current conserved state: five arrays over the
+ * interior only, C order (n[0], n[1], n[2]).  Valid until the next step. */
+int dg_device_state(dg_ctx *ctx, void *ptrs[5]);
+
+/* Timing harness: runs nsteps plain (non-graph) steps on the context's stream
+ * with a HIP event pair around every launch; reports the mean device time of
+ * the fused step kernel and of the per-step reduction kernel, and the total
+ * wall time of the nsteps steps (also measured with events). */
+int dg_time_steps(dg_ctx *ctx, int64_t nsteps, double *ms_total, double *ms_step_kernel,
+                  double *ms_reduce_kernel);
+
+/* Library and kernel facts for reporting. */
+int dg_kernel_info(dg_ctx *ctx, int *tile_y, int *chunk_x, int64_t *workgroups,
+                   int *lds_bytes, int *threads_per_workgroup);
+
+int dg_device_count(int *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DEEPGREEN_H */

@@ -1,0 +1,128 @@
+// CPU emulation of the small HIP surface used by deep-green_amd/csrc.
+// TEST INFRASTRUCTURE ONLY: lets g++ compile the HIP kernels for the host so
+// their logic (indexing, barriers, shuffles, boundary rules) can be checked
+// bit for bit against the oracle on machines without a GPU.  Each workgroup
+// runs as blockDim host threads with a real barrier; blocks run in sequence.
+#pragma once
+#include <pthread.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct emu_idx { unsigned x = 0, y = 0, z = 0; };
+extern thread_local emu_idx threadIdx, blockIdx;
+extern emu_idx blockDim, gridDim;
+
+namespace emu {
+extern pthread_barrier_t bar;
+extern double xbuf[1024];
+inline void sync() { pthread_barrier_wait(&bar); }
+template <typename T> T exch(T v, int src_lane_delta, int mode)
+{
+    const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid & ~63;
+    sync();
+    xbuf[tid] = (double)v;
+    std::memcpy(&xbuf[tid], &v, sizeof(T));
+    sync();
+    int src;
+    if (mode == 0) src = lane + src_lane_delta < 64 ? lane + src_lane_delta : lane;   // down
+    else if (mode == 1) src = lane - src_lane_delta >= 0 ? lane - src_lane_delta : lane; // up
+    else src = lane ^ src_lane_delta;                                                    // xor
+    T out;
+    std::memcpy(&out, &xbuf[wave + src], sizeof(T));
+    sync();
+    return out;
+}
+}  // namespace emu
+
+inline void __syncthreads() { emu::sync(); }
+template <typename T> inline T __shfl_down(T v, unsigned d) { return emu::exch<T>(v, (int)d, 0); }
+template <typename T> inline T __shfl_up(T v, unsigned d) { return emu::exch<T>(v, (int)d, 1); }
+template <typename T> inline T __shfl_xor(T v, int m) { return emu::exch<T>(v, m, 2); }
+inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+
+typedef int hipError_t;
+typedef void *hipStream_t;
+typedef void *hipEvent_t;
+typedef void *hipGraph_t;
+typedef void *hipGraphExec_t;
+enum { hipSuccess = 0, hipErrorUnknown = 999 };
+enum { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipStreamNonBlocking = 1,
+       hipHostMallocDefault = 0, hipStreamCaptureModeThreadLocal = 0 };
+typedef int hipMemcpyKind;
+
+namespace emu {
+void launch(unsigned grid, unsigned block, const std::function<void()> &body);
+extern bool capturing;
+extern std::vector<std::function<void()>> captured;
+}  // namespace emu
+
+#define hipLaunchKernelGGL(K, G, B, SH, S, ...)                                          \
+    do {                                                                                 \
+        auto emu_body_ = [=]() { K(__VA_ARGS__); };                                      \
+        const unsigned g_ = dim3(G).x, b_ = dim3(B).x;                                   \
+        if (emu::capturing) emu::captured.push_back([=]() { emu::launch(g_, b_, emu_body_); }); \
+        else emu::launch(g_, b_, emu_body_);                                             \
+    } while (0)
+
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline const char *hipGetErrorString(hipError_t) { return "emulated"; }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
+inline hipError_t hipMalloc(void **p, size_t n) { *p = std::calloc(1, n ? n : 1); return *p ? hipSuccess : 2; }
+template <typename T> inline hipError_t hipMalloc(T **p, size_t n) { return hipMalloc((void **)p, n); }
+inline hipError_t hipFree(void *p) { std::free(p); return hipSuccess; }
+inline hipError_t hipHostMalloc(void **p, size_t n, int) { return hipMalloc(p, n); }
+template <typename T> inline hipError_t hipHostMalloc(T **p, size_t n, int f) { return hipHostMalloc((void **)p, n, f); }
+inline hipError_t hipHostFree(void *p) { std::free(p); return hipSuccess; }
+inline hipError_t hipMemcpy(void *d, const void *s, size_t n, int) { std::memcpy(d, s, n); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, int, hipStream_t)
+{
+    if (emu::capturing) return hipErrorUnknown;
+    std::memcpy(d, s, n);
+    return hipSuccess;
+}
+inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, int) { *s = (void *)1; return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t *e) { *e = (void *)1; return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0; return hipSuccess; }
+inline hipError_t hipStreamBeginCapture(hipStream_t, int) { emu::capturing = true; emu::captured.clear(); return hipSuccess; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g)
+{
+    emu::capturing = false;
+    *g = new std::vector<std::function<void()>>(emu::captured);
+    return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, void *, void *, size_t)
+{
+    *e = new std::vector<std::function<void()>>(*static_cast<std::vector<std::function<void()>> *>(g));
+    return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) { delete static_cast<std::vector<std::function<void()>> *>(g); return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t g) { delete static_cast<std::vector<std::function<void()>> *>(g); return hipSuccess; }
+inline hipError_t hipGraphLaunch(hipGraphExec_t g, hipStream_t)
+{
+    for (auto &f : *static_cast<std::vector<std::function<void()>> *>(g)) f();
+    return hipSuccess;
+}
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }

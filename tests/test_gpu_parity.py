@@ -1,0 +1,204 @@
+"""GPU parity: the HIP step library (through its C ABI) against the golden
+vectors written by the reference's Fortran physics and against the oracle.
+
+Exact math is checked bit for bit: FP64 against the real(8) reference build,
+FP32 against the real(4) build (whose arithmetic equals the Rust f32
+solver's).  Cases the Fortran cannot represent (the Rust gravity height,
+upper walls, limiters, partial tiles) are checked bit for bit against the
+oracle, itself pinned by tests/test_oracle_*.py.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+from dg_oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def lib_run(c, precision, math="exact", limiter="none", tile_y=0, chunk_x=0, per_step=True, params=None):
+    from deepgreen import Simulation, Solver
+    prm = dict(cases.PARAMS[params or c["params"]])
+    prm.pop("gravity_k_shift")
+    sim = Simulation(*c["interior"], c["ds"], c["bcs"], precision=precision, math=math, limiter=limiter,
+                     tile_y=tile_y, chunk_x=chunk_x, **prm)
+    _say(f"  lib {c['interior']} {precision}/{math}/{limiter} ty={tile_y} lx={chunk_x} bcs={c['bcs']}")
+    with Solver(sim) as s:
+        s.set_primitives(c["w"])
+        if per_step:
+            dts = np.array([s.compute_step() for _ in range(c["nsteps"])])
+        else:
+            t, n = s.run(c["nsteps"])
+            assert n == c["nsteps"]
+            dts = None
+        U = np.stack([np.array(s.interior(a)) for a in s.get_state()])
+        sums = s.conservation_sums()
+    return U, dts, sums
+
+
+def oracle_run(c, precision, limiter="none", params=None):
+    prm = dict(cases.PARAMS[params or c["params"]])
+    prm["gravity_k_shift"] = 0  # the library follows the Rust height (physics.rs:368)
+    o = Oracle(c["interior"], c["ds"], c["bcs"], precision="f64" if precision == "fp64" else "f32",
+               limiter=limiter, nthreads=4, **prm)
+    o.set_primitives(c["w"])
+    n, _, dts = o.compute(1e30, c["nsteps"])
+    return np.stack([np.array(o.interior_view(a)) for a in o.cons]), dts
+
+
+CFG_CASES = [n for n in cases.CASES if cases.CASES[n][5] == "cfg"]
+
+
+@pytest.mark.parametrize("name", CFG_CASES)
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_matches_reference_fortran(golden_dir, name, precision):
+    _say(f"case {name} {precision}")
+    c = cases.build_case(name)
+    g = np.load(os.path.join(golden_dir, f"{name}.npz"))
+    key = "dp" if precision == "fp64" else "sp"
+    U, dts, _ = lib_run(c, precision)
+    np.testing.assert_array_equal(dts, g[f"{key}_dt"].astype(np.float64))
+    if name == "sod64":
+        mid = [s // 2 for s in U.shape[1:]]
+        np.testing.assert_array_equal(U[:, :, mid[1], mid[2]], g[f"{key}_line"])
+    else:
+        np.testing.assert_array_equal(U, g[f"{key}_U"])
+
+
+@pytest.mark.parametrize("name", ["smooth_lowerwalls_native", "smooth_periodic_native"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_gravity_and_walls_match_oracle(name, precision):
+    _say(f"case {name} {precision}")
+    c = cases.build_case(name)
+    U, dts, _ = lib_run(c, precision)
+    Uo, dto = oracle_run(c, precision)
+    np.testing.assert_array_equal(dts, dto.astype(np.float64))
+    np.testing.assert_array_equal(U, Uo)
+
+
+def _walls_case():
+    w, ds = cases.smooth_random((10, 9, 11), seed=21, amp_v=0.1, p_base=100.0)
+    return dict(w=w, ds=ds, interior=(10, 9, 11), bcs=[cases.WALL] * 6, nsteps=12, params="native")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_all_walls_match_oracle(precision):
+    c = _walls_case()
+    U, dts, _ = lib_run(c, precision)
+    Uo, dto = oracle_run(c, precision)
+    np.testing.assert_array_equal(dts, dto.astype(np.float64))
+    np.testing.assert_array_equal(U, Uo)
+
+
+@pytest.mark.parametrize("limiter", ["minmod", "vanleer"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_limited_blast_matches_oracle(limiter, precision):
+    w, ds = cases.blast((20, 18, 22), radius=0.2, p_in=10.0, p_out=0.1)
+    c = dict(w=w, ds=ds, interior=(20, 18, 22), bcs=[cases.PERIODIC] * 2 + [cases.OUTFLOW] * 2 + [cases.WALL] * 2,
+             nsteps=15, params="cfg")
+    U, dts, _ = lib_run(c, precision, limiter=limiter)
+    Uo, dto = oracle_run(c, precision, limiter=limiter)
+    np.testing.assert_array_equal(dts, dto.astype(np.float64))
+    np.testing.assert_array_equal(U, Uo)
+
+
+@pytest.mark.parametrize("tile_y,chunk_x", [(4, 3), (8, 5), (16, 64), (8, 1)])
+def test_tiling_does_not_change_bits(tile_y, chunk_x):
+    """Partial tiles in y and z, x chunks of every length: same bits."""
+    w, ds = cases.smooth_random((21, 19, 70), seed=8)
+    c = dict(w=w, ds=ds, interior=(21, 19, 70), bcs=[cases.PERIODIC] * 2 + [cases.OUTFLOW] * 4, nsteps=6,
+             params="cfg")
+    U, dts, _ = lib_run(c, "fp32", tile_y=tile_y, chunk_x=chunk_x)
+    Uo, dto = oracle_run(c, "fp32")
+    np.testing.assert_array_equal(U, Uo)
+    np.testing.assert_array_equal(dts, dto.astype(np.float64))
+
+
+def test_graph_run_equals_single_steps():
+    c = cases.build_case("smooth_periodic")
+    c["nsteps"] = 37  # two graph replays of 16 plus 5 single launches
+    U1, _, s1 = lib_run(c, "fp32", per_step=True)
+    U2, _, s2 = lib_run(c, "fp32", per_step=False)
+    np.testing.assert_array_equal(U1, U2)
+    np.testing.assert_array_equal(s1, s2)
+
+
+def test_conservation_sums_and_determinism():
+    c = cases.build_case("blast16")
+    U1, _, s1 = lib_run(c, "fp64", per_step=False)
+    U2, _, s2 = lib_run(c, "fp64", per_step=False)
+    np.testing.assert_array_equal(U1, U2)
+    np.testing.assert_array_equal(s1, s2)
+    np.testing.assert_allclose(s1, U1.reshape(5, -1).sum(1), rtol=1e-13, atol=1e-13)
+
+
+def test_compute_with_total_time_matches_reference_loop():
+    """main.rs:283-299: stop when tnow > total_time (f32 clock for FP32)."""
+    from deepgreen import Simulation, Solver
+    c = cases.build_case("smooth_mixed")
+    prm = dict(cases.PARAMS["cfg"])
+    prm.pop("gravity_k_shift")
+    o = Oracle(c["interior"], c["ds"], c["bcs"], precision="f32", **cases.PARAMS["cfg"])
+    o.set_primitives(c["w"])
+    n_o, t_o, _ = o.compute(0.1, 1000)
+    sim = Simulation(*c["interior"], c["ds"], c["bcs"], precision="fp32", **prm)
+    with Solver(sim) as s:
+        s.set_primitives(c["w"])
+        t, n = s.compute(0.1, 1000)
+        U = np.stack([np.array(s.interior(a)) for a in s.get_state()])
+    assert n == n_o
+    assert np.float32(t) == np.float32(t_o)
+    np.testing.assert_array_equal(U, np.stack([np.array(o.interior_view(a)) for a in o.cons]))
+
+
+def test_nan_wave_speed_falls_back_to_cfl():
+    from deepgreen import Simulation, Solver
+    c = cases.build_case("smooth_periodic")
+    c["w"][4][5, 5, 5] = -5.0  # negative pressure in one interior cell
+    prm = dict(cases.PARAMS["cfg"])
+    prm.pop("gravity_k_shift")
+    sim = Simulation(*c["interior"], c["ds"], c["bcs"], precision="fp32", **prm)
+    with Solver(sim) as s:
+        s.set_primitives(c["w"])
+        assert s.compute_step() == float(np.float32(0.4) * np.float32(1.0))
+
+
+def test_get_state_ghosts_follow_boundary_maps():
+    from deepgreen import Simulation, Solver, PRIMITIVE
+    c = cases.build_case("smooth_lowerwalls_native")
+    prm = dict(cases.PARAMS["native"])
+    prm.pop("gravity_k_shift")
+    sim = Simulation(*c["interior"], c["ds"], c["bcs"], precision="fp64", **prm)
+    o = Oracle(c["interior"], c["ds"], c["bcs"], precision="f64", **cases.PARAMS["native"])
+    o.set_primitives(c["w"])
+    o.calc_conserved()
+    o.calc_primitives()
+    o.update_primitive_ghost_cells()
+    with Solver(sim) as s:
+        s.set_primitives(c["w"])
+        W = s.get_state(PRIMITIVE)
+    # faces and edges: every cell whose ghost status is along at most two axes
+    for a, b in zip(W, o.prim):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_fp32_bulk_and_fast_errors_against_fp64():
+    """FP32-bulk and FP32-fast against the FP64 oracle (limited blast): the
+    errors are reported and bounded."""
+    w, ds = cases.blast((32, 32, 32), radius=0.2, p_in=10.0, p_out=0.1)
+    c = dict(w=w, ds=ds, interior=(32, 32, 32), bcs=[cases.PERIODIC] * 6, nsteps=30, params="cfg")
+    Uo, _ = oracle_run(c, "fp64", limiter="minmod")
+    for prec, math in (("fp32_bulk", "exact"), ("fp32", "exact"), ("fp32", "fast"), ("fp32_bulk", "fast")):
+        U, _, _ = lib_run(c, prec, math=math, limiter="minmod", per_step=False)
+        l1 = np.abs(U - Uo).reshape(5, -1).sum(1) / np.abs(Uo).reshape(5, -1).sum(1)
+        linf = np.abs(U - Uo).reshape(5, -1).max(1) / np.abs(Uo).reshape(5, -1).max(1)
+        _say(f"  {prec}/{math}: rel L1 {l1}, rel Linf {linf}")
+        assert np.all(l1 < 1e-4) and np.all(linf < 1e-2)
