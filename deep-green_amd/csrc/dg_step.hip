@@ -612,7 +612,10 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
                 T rb[5], lb[5], Fb[5];
 #pragma unroll
                 for (int f = 0; f < 5; ++f) { rb[f] = L.Zp[f][tz]; lb[f] = L.Z0[f][tz]; }
-                const T hb = zgrav ? grav_h<T>(z0 - 1, nz, C.ds) : T(0);
+                // periodic: the reference copies this face from face (n-1, n)
+                // (boundaries.rs:174-178 on fluxes), gravity height included
+                const int zh = (z0 == 0 && A.bc[4] == BC_PERIODIC) ? nz - 1 : z0 - 1;
+                const T hb = zgrav ? grav_h<T>(zh, nz, C.ds) : T(0);
                 face_flux<T, FAST>(lb, rb, 2, C, zgrav, hb, Fb);
 #pragma unroll
                 for (int f = 0; f < 5; ++f) L.FZb[f][tz] = Fb[f];

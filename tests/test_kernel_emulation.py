@@ -77,3 +77,13 @@ def test_emulated_fast_math_close_to_exact(emu_lib):
     w, ds = cases.smooth_random(interior, seed=4)
     U, _, Uo, _ = run_both(emu_lib, interior, w, ds, [cases.PERIODIC] * 6, 2, "fp32", math="fast")
     np.testing.assert_allclose(U, Uo, rtol=2e-5, atol=1e-6 * np.abs(Uo).max())
+
+
+def test_emulated_periodic_gravity(emu_lib):
+    """Periodic z with gravity: the lower boundary z-flux is the copy of the
+    face (n-1, n), whose gravity height belongs to z = n-1."""
+    interior = (5, 6, 8)
+    w, ds = cases.smooth_random(interior, seed=6, amp_v=0.1, p_base=100.0)
+    U, dts, Uo, dto = run_both(emu_lib, interior, w, ds, [cases.PERIODIC] * 6, 2, "fp64", params="native")
+    np.testing.assert_array_equal(dts, dto.astype(np.float64))
+    np.testing.assert_array_equal(U, Uo)
