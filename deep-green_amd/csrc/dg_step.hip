@@ -226,13 +226,14 @@ __device__ __forceinline__ int rot_field(int axis, int slot)
 template <typename T>
 struct Consts {
     T ds, gamma, gm1, V, invV, two_ds, inv_two_ds, half_ds, half_dt, dtds, G;
-    int lim;
 };
 
 // LDS layout of the step kernel.
 template <typename T, int TY>
 struct StepLDS {
-    T W[3][5][TY + 4][RZ];   // primitives ring: planes t-1, t, t+1 (by x mod 3)
+    static constexpr int FS = (TY + 4) * RZ;  // one field of one plane
+    static constexpr int PS = 5 * FS;         // one plane
+    T W[3 * PS];             // primitives ring [plane t mod 3][field][row][col]
     T Ym[5][TY + 1][TZ];     // -y face state of rows 0..TY
     T Yb[5][TZ];             // flux through the tile's lower y face
     T FY[5][TY][TZ];         // flux through face (row, row+1)
@@ -242,36 +243,60 @@ struct StepLDS {
     T FZb[5][TY];            // flux through the tile's lower z face
 };
 
-// Slopes and Hancock-extrapolated state of the cell at LDS region position
-// (ry, rz) of plane slot s0 (neighbours in slots sm, sp).  g[f*3+d] holds the
-// slope (gradient) of field f along axis d.
-template <typename T, int TY, bool FAST>
-__device__ __forceinline__ void cell_xtr(const StepLDS<T, TY> &L, int sm, int s0, int sp, int ry, int rz,
-                                         const Consts<T> &C, T xtr[5], T g[15])
+// Slope of one field along one axis from the primitives m (at -1), c, p (at
+// +1).  Exact math: the gradient of calc_gradients (physics.rs:206-208) or
+// the limited slope / ds.  Fast math: the same scaled by 2 ds (the factor is
+// folded into the constants that consume it).
+template <typename T, bool FAST, int LIM>
+__device__ __forceinline__ T slope(T m, T c, T p, const Consts<T> &C)
 {
+    if constexpr (LIM == LIM_NONE) {
+        if constexpr (!FAST) return (p - m) / C.two_ds;
+        else return p - m;
+    } else {
+        if constexpr (!FAST) return limited<T>(c - m, p - c, LIM) / C.ds;
+        else return T(2) * limited<T>(c - m, p - c, LIM);
+    }
+}
+
+// Where a tile cell takes its state from (precomputed once per thread: the
+// mapping does not depend on the plane).  r: flat region index (row*RZ+col)
+// whose slopes and extrapolated state are used; for a wall ghost along y
+// (mask bit 0) or z (bit 1) the normal velocity of xtr flips and the normal
+// slope is evaluated at the ghost's own position ry_wall / rz_wall.
+struct EvalDesc { int r, ry_wall, rz_wall, mask; };
+
+__device__ __forceinline__ EvalDesc make_desc(int py, int pz, int y0, int z0, const StepArgs &A, int ty_max)
+{
+    const int qy = eval_local(py, y0, A.n[1], A.bc[2], A.bc[3], ty_max);
+    const int qz = eval_local(pz, z0, A.n[2], A.bc[4], A.bc[5], TZ);
+    EvalDesc d;
+    d.r = (qy + 2) * RZ + (qz + 2);
+    d.mask = 0;
+    d.ry_wall = d.rz_wall = d.r;
+    if (wall_ghost(py, y0, A.n[1], A.bc[2], A.bc[3])) { d.mask |= 1; d.ry_wall = (py + 2) * RZ + (qz + 2); }
+    if (wall_ghost(pz, z0, A.n[2], A.bc[4], A.bc[5])) { d.mask |= 2; d.rz_wall = (qy + 2) * RZ + (pz + 2); }
+    return d;
+}
+
+// Slopes and Hancock-extrapolated state (extrapolate_primitives_t,
+// physics.rs:217-282) of a cell described by `d`, from ring slots sm, s0, sp.
+// g[f*3+d] is the slope of field f along axis d.
+template <typename T, int TY, bool FAST, int LIM>
+__device__ __forceinline__ void eval_cell(const StepLDS<T, TY> &L, int sm, int s0, int sp, const EvalDesc &d,
+                                          const Consts<T> &C, T xtr[5], T g[15])
+{
+    constexpr int FS = StepLDS<T, TY>::FS, PS = StepLDS<T, TY>::PS;
+    const T *Wm = L.W + sm * PS, *W0 = L.W + s0 * PS, *Wp = L.W + sp * PS;
     T w[5];
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
-        const T c = L.W[s0][f][ry][rz];
-        const T xm = L.W[sm][f][ry][rz], xp = L.W[sp][f][ry][rz];
-        const T ym = L.W[s0][f][ry - 1][rz], yp = L.W[s0][f][ry + 1][rz];
-        const T zm = L.W[s0][f][ry][rz - 1], zp = L.W[s0][f][ry][rz + 1];
+        const int o = f * FS + d.r;
+        const T c = W0[o];
         w[f] = c;
-        if (C.lim == LIM_NONE) {
-            if constexpr (!FAST) {
-                g[f * 3 + 0] = (xp - xm) / C.two_ds;
-                g[f * 3 + 1] = (yp - ym) / C.two_ds;
-                g[f * 3 + 2] = (zp - zm) / C.two_ds;
-            } else {
-                g[f * 3 + 0] = (xp - xm) * C.inv_two_ds;
-                g[f * 3 + 1] = (yp - ym) * C.inv_two_ds;
-                g[f * 3 + 2] = (zp - zm) * C.inv_two_ds;
-            }
-        } else {
-            g[f * 3 + 0] = limited<T>(c - xm, xp - c, C.lim) / C.ds;
-            g[f * 3 + 1] = limited<T>(c - ym, yp - c, C.lim) / C.ds;
-            g[f * 3 + 2] = limited<T>(c - zm, zp - c, C.lim) / C.ds;
-        }
+        g[f * 3 + 0] = slope<T, FAST, LIM>(Wm[o], c, Wp[o], C);
+        g[f * 3 + 1] = slope<T, FAST, LIM>(W0[o - RZ], c, W0[o + RZ], C);
+        g[f * 3 + 2] = slope<T, FAST, LIM>(W0[o - 1], c, W0[o + 1], C);
     }
     const T r = w[0], ux = w[1], uy = w[2], uz = w[3], p = w[4];
     const T drx = g[0], dry = g[1], drz = g[2];
@@ -281,13 +306,14 @@ __device__ __forceinline__ void cell_xtr(const StepLDS<T, TY> &L, int sm, int s0
     const T dpx = g[12], dpy = g[13], dpz = g[14];
     const T hd = C.half_dt;
     if constexpr (!FAST) {
-        // extrapolate_primitives_t (physics.rs:251-278), same association
+        // physics.rs:251-278, same association
         xtr[0] = r - hd * (ux * drx + r * dxx + uy * dry + r * dyy + uz * drz + r * dzz);
         xtr[1] = ux - hd * (ux * dxx + uy * dxy + uz * dxz + (T(1.0) / r) * dpx);
         xtr[2] = uy - hd * (ux * dyx + uy * dyy + uz * dyz + (T(1.0) / r) * dpy);
         xtr[3] = uz - hd * (ux * dzx + uy * dzy + uz * dzz + (T(1.0) / r) * dpz);
         xtr[4] = p - hd * (C.gamma * p * (dxx + dyy + dzz) + ux * dpx + uy * dpy + uz * dpz);
     } else {
+        // hd here is dt / 2 / (2 ds): slopes are undivided differences
         const T ir = drcp<T>(r);
         const T div = dxx + dyy + dzz;
         xtr[0] = dfma<T>(-hd, dfma<T>(uz, drz, dfma<T>(uy, dry, dfma<T>(ux, drx, r * div))), r);
@@ -296,21 +322,22 @@ __device__ __forceinline__ void cell_xtr(const StepLDS<T, TY> &L, int sm, int s0
         xtr[3] = dfma<T>(-hd, dfma<T>(ir, dpz, dfma<T>(uz, dzz, dfma<T>(uy, dzy, ux * dzx))), uz);
         xtr[4] = dfma<T>(-hd, dfma<T>(uz, dpz, dfma<T>(uy, dpy, dfma<T>(ux, dpx, C.gamma * p * div))), p);
     }
-}
-
-// Normal slope along axis d at region position (ry, rz) of slot s0 (used for
-// a wall ghost, whose primitives in the ring are the mirrored ones).
-template <typename T, int TY>
-__device__ __forceinline__ T slope_at(const StepLDS<T, TY> &L, int sm, int s0, int sp, int ry, int rz,
-                                      int f, int d, const Consts<T> &C)
-{
-    const T c = L.W[s0][f][ry][rz];
-    T m, p;
-    if (d == 0) { m = L.W[sm][f][ry][rz]; p = L.W[sp][f][ry][rz]; }
-    else if (d == 1) { m = L.W[s0][f][ry - 1][rz]; p = L.W[s0][f][ry + 1][rz]; }
-    else { m = L.W[s0][f][ry][rz - 1]; p = L.W[s0][f][ry][rz + 1]; }
-    if (C.lim == LIM_NONE) return (p - m) / C.two_ds;
-    return limited<T>(c - m, p - c, C.lim) / C.ds;
+    if (d.mask & 1) {   // wall ghost along y: mirrored xtr, slope on the ghost
+        xtr[2] = -xtr[2];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            const int o = f * FS + d.ry_wall;
+            g[f * 3 + 1] = slope<T, FAST, LIM>(W0[o - RZ], W0[o], W0[o + RZ], C);
+        }
+    }
+    if (d.mask & 2) {
+        xtr[3] = -xtr[3];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            const int o = f * FS + d.rz_wall;
+            g[f * 3 + 2] = slope<T, FAST, LIM>(W0[o - 1], W0[o], W0[o + 1], C);
+        }
+    }
 }
 
 // Face state of a cell along axis d: xtr + sign * slope * ds/2 (physics.rs:313-336).
@@ -320,30 +347,12 @@ __device__ __forceinline__ void face_state(const T xtr[5], const T g[15], int d,
 {
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
-        const T q = g[f * 3 + d] * C.half_ds;
-        s[f] = plus ? xtr[f] + q : xtr[f] - q;
-    }
-}
-
-// Evaluate the state of tile cell (py, pz) of the current plane per the
-// boundary rules, returning its extrapolated state and slopes.
-template <typename T, int TY, bool FAST>
-__device__ __forceinline__ void eval_cell(const StepLDS<T, TY> &L, int sm, int s0, int sp, int py, int pz,
-                                          int y0, int z0, const StepArgs &A, const Consts<T> &C, T xtr[5],
-                                          T g[15])
-{
-    const int qy = eval_local(py, y0, A.n[1], A.bc[2], A.bc[3], TY);
-    const int qz = eval_local(pz, z0, A.n[2], A.bc[4], A.bc[5], TZ);
-    cell_xtr<T, TY, FAST>(L, sm, s0, sp, qy + 2, qz + 2, C, xtr, g);
-    if (wall_ghost(py, y0, A.n[1], A.bc[2], A.bc[3])) {
-        xtr[2] = -xtr[2];
-#pragma unroll
-        for (int f = 0; f < 5; ++f) g[f * 3 + 1] = slope_at<T, TY>(L, sm, s0, sp, py + 2, qz + 2, f, 1, C);
-    }
-    if (wall_ghost(pz, z0, A.n[2], A.bc[4], A.bc[5])) {
-        xtr[3] = -xtr[3];
-#pragma unroll
-        for (int f = 0; f < 5; ++f) g[f * 3 + 2] = slope_at<T, TY>(L, sm, s0, sp, qy + 2, pz + 2, f, 2, C);
+        if constexpr (!FAST) {
+            const T q = g[f * 3 + d] * C.half_ds;
+            s[f] = plus ? xtr[f] + q : xtr[f] - q;
+        } else {
+            s[f] = dfma<T>(plus ? C.half_ds : -C.half_ds, g[f * 3 + d], xtr[f]);
+        }
     }
 }
 
@@ -370,19 +379,28 @@ __device__ __forceinline__ T grav_h(int z, int nz, T ds)
 
 __device__ __forceinline__ int ring(int x) { return (x + 3) % 3; }  // x >= -3
 
-// Load plane xg's primitives (tile plus two-cell halo) into ring slot `slot`.
-// The thread's own cell's conserved values are returned in u_own.
-template <typename T, int TY, bool FAST>
-__device__ __forceinline__ void load_plane(StepLDS<T, TY> &L, int slot, int xg, int y0, int z0, const StepArgs &A,
-                                           const Fields<const T> &U, const Consts<T> &C, int tid)
+// Region cells (tile plus two-cell halo) a thread loads for every plane:
+// flat LDS index, (y, z) offset of the mapped source cell, and flips.
+template <int TY>
+struct LoadPlan {
+    static constexpr int NT = TY * TZ;
+    static constexpr int NREG = (TY + 4) * RZ;
+    static constexpr int NS = (NREG + NT - 1) / NT;
+    int lds[NS];
+    int off[NS];
+    int flip[NS];   // bit 1: y, bit 2: z
+};
+
+template <int TY>
+__device__ __forceinline__ void make_plan(LoadPlan<TY> &P, int tid, int y0, int z0, const StepArgs &A)
 {
-    constexpr int NT = TY * TZ;
-    constexpr int NHALO = 4 * RZ + 4 * TY;
-    const AxisMap mx = map_axis(xg, A.n[0], A.bc[0], A.bc[1]);
-    const size_t plane = (size_t)mx.idx * A.n[1] * A.n[2];
-    for (int c = tid; c < NT + NHALO; c += NT) {
+    constexpr int NT = LoadPlan<TY>::NT, NREG = LoadPlan<TY>::NREG;
+#pragma unroll
+    for (int q = 0; q < LoadPlan<TY>::NS; ++q) {
+        const int c = tid + q * NT;
+        if (c >= NREG) { P.lds[q] = -1; P.off[q] = 0; P.flip[q] = 0; continue; }
         int ry, rz;
-        if (c < NT) { ry = (c >> 6) + 2; rz = (c & 63) + 2; }
+        if (c < NT) { ry = (c >> 6) + 2; rz = (c & 63) + 2; }   // the thread's own cell first
         else {
             const int h = c - NT;
             if (h < 4 * RZ) {
@@ -398,16 +416,35 @@ __device__ __forceinline__ void load_plane(StepLDS<T, TY> &L, int slot, int xg, 
         }
         const AxisMap my = map_axis(y0 + ry - 2, A.n[1], A.bc[2], A.bc[3]);
         const AxisMap mz = map_axis(z0 + rz - 2, A.n[2], A.bc[4], A.bc[5]);
-        const size_t idx = plane + (size_t)my.idx * A.n[2] + mz.idx;
+        P.lds[q] = ry * RZ + rz;
+        P.off[q] = my.idx * A.n[2] + mz.idx;
+        P.flip[q] = (my.flip ? 2 : 0) | (mz.flip ? 4 : 0);
+    }
+}
+
+// Load plane xg's primitives into ring slot `slot` (calc_primitives on the
+// mapped cells; ghost positions take the boundary maps).
+template <typename T, int TY, bool FAST>
+__device__ __forceinline__ void load_plane(StepLDS<T, TY> &L, int slot, int xg, const LoadPlan<TY> &P,
+                                           const StepArgs &A, const Fields<const T> &U, const Consts<T> &C)
+{
+    constexpr int FS = StepLDS<T, TY>::FS, PS = StepLDS<T, TY>::PS;
+    const AxisMap mx = map_axis(xg, A.n[0], A.bc[0], A.bc[1]);
+    const size_t plane = (size_t)mx.idx * A.n[1] * A.n[2];
+    T *Ws = L.W + slot * PS;
+#pragma unroll
+    for (int q = 0; q < LoadPlan<TY>::NS; ++q) {
+        if (P.lds[q] < 0) continue;
+        const size_t idx = plane + (size_t)P.off[q];
         T u[5], w[5];
 #pragma unroll
         for (int f = 0; f < 5; ++f) u[f] = U.f[f][idx];
         to_prim<T, FAST>(u, C.V, C.invV, C.gm1, w);
         if (mx.flip) w[1] = -w[1];
-        if (my.flip) w[2] = -w[2];
-        if (mz.flip) w[3] = -w[3];
+        if (P.flip[q] & 2) w[2] = -w[2];
+        if (P.flip[q] & 4) w[3] = -w[3];
 #pragma unroll
-        for (int f = 0; f < 5; ++f) L.W[slot][f][ry][rz] = w[f];
+        for (int f = 0; f < 5; ++f) Ws[f * FS + P.lds[q]] = w[f];
     }
 }
 
@@ -443,7 +480,7 @@ __device__ __forceinline__ void block_partials(double den, int nanf, const doubl
     }
 }
 
-template <typename T, int TY, bool FAST, bool BULK>
+template <typename T, int TY, bool FAST, bool BULK, int LIM>
 __global__ void __launch_bounds__(TY * TZ)
 step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, const Scalars *__restrict__ sc,
             double *__restrict__ partials)
@@ -485,11 +522,10 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
     C.invV = T(1) / C.V;
     C.two_ds = T(2.0) * C.ds;
     C.inv_two_ds = T(1) / C.two_ds;
-    C.half_ds = C.ds / T(2.);
+    C.half_ds = FAST ? T(0.25) : C.ds / T(2.);   // fast: slopes are scaled by 2 ds
     C.G = T(A.g);
-    C.lim = A.limiter;
     const T dt = T(sc->dt);
-    C.half_dt = T(0.5) * dt;
+    C.half_dt = FAST ? T(0.5) * dt * C.inv_two_ds : T(0.5) * dt;
     C.dtds = FAST ? dt * C.ds * C.ds * T(0.5) : dt * C.ds;
 
     double sums[5] = {0, 0, 0, 0, 0};
@@ -511,20 +547,29 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
 #pragma unroll
     for (int f = 0; f < 5; ++f) { xprev[f] = gxprev[f] = fxlo[f] = fyu[f] = fyl[f] = fzu[f] = fzl[f] = T(0); }
 
-    load_plane<T, TY, FAST>(L, ring(x0 - 2), x0 - 2, y0, z0, A, U, C, tid);
-    load_plane<T, TY, FAST>(L, ring(x0 - 1), x0 - 1, y0, z0, A, U, C, tid);
+    // plane-independent addressing, evaluated once
+    LoadPlan<TY> plan;
+    make_plan<TY>(plan, tid, y0, z0, A);
+    const EvalDesc own = make_desc(ty, tz, y0, z0, A, TY);
+    EvalDesc halo = own;
+    if (ty == 0) halo = make_desc(-1, tz, y0, z0, A, TY);
+    else if (ty == 1) halo = make_desc(TY, tz, y0, z0, A, TY);
+    else if (ty == 2 && tz < 2 * TY) halo = make_desc(tz < TY ? tz : tz - TY, tz < TY ? -1 : TZ, y0, z0, A, TY);
+
+    load_plane<T, TY, FAST>(L, ring(x0 - 2), x0 - 2, plan, A, U, C);
+    load_plane<T, TY, FAST>(L, ring(x0 - 1), x0 - 1, plan, A, U, C);
 
     const bool zgrav = A.gravity != 0;
     for (int t = x0 - 1; t <= xend; ++t) {
         const bool full = t >= x0 && t < xend;
         const int sm = ring(t - 1), s0 = ring(t), sp = ring(t + 1);
         // ---- A: primitives of plane t+1 into the ring
-        load_plane<T, TY, FAST>(L, sp, t + 1, y0, z0, A, U, C, tid);
+        load_plane<T, TY, FAST>(L, sp, t + 1, plan, A, U, C);
         __syncthreads();
 
         // ---- B: extrapolated state and slopes of the own cell
         T xtr[5], g[15];
-        eval_cell<T, TY, FAST>(L, sm, s0, sp, ty, tz, y0, z0, A, C, xtr, g);
+        eval_cell<T, TY, FAST, LIM>(L, sm, s0, sp, own, C, xtr, g);
         T szm[5];
         face_state<T, FAST>(xtr, g, 2, false, C, szm);
         if (full) {
@@ -539,14 +584,14 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
             // halo states
             if (ty == 0) {
                 T hx[5], hg[15], hs[5], F[5];
-                eval_cell<T, TY, FAST>(L, sm, s0, sp, -1, tz, y0, z0, A, C, hx, hg);
+                eval_cell<T, TY, FAST, LIM>(L, sm, s0, sp, halo, C, hx, hg);
                 face_state<T, FAST>(hx, hg, 1, true, C, hs);
                 face_flux<T, FAST>(sym, hs, 1, C, false, T(0), F);   // face (-1, 0)
 #pragma unroll
                 for (int f = 0; f < 5; ++f) L.Yb[f][tz] = F[f];
             } else if (ty == 1) {
                 T hx[5], hg[15], hs[5];
-                eval_cell<T, TY, FAST>(L, sm, s0, sp, TY, tz, y0, z0, A, C, hx, hg);
+                eval_cell<T, TY, FAST, LIM>(L, sm, s0, sp, halo, C, hx, hg);
                 face_state<T, FAST>(hx, hg, 1, false, C, hs);
 #pragma unroll
                 for (int f = 0; f < 5; ++f) L.Ym[f][TY][tz] = hs[f];
@@ -554,7 +599,7 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
                 const bool lowz = tz < TY;
                 const int row = lowz ? tz : tz - TY;
                 T hx[5], hg[15], hs[5];
-                eval_cell<T, TY, FAST>(L, sm, s0, sp, row, lowz ? -1 : TZ, y0, z0, A, C, hx, hg);
+                eval_cell<T, TY, FAST, LIM>(L, sm, s0, sp, halo, C, hx, hg);
                 face_state<T, FAST>(hx, hg, 2, lowz, C, hs);
                 if (lowz) {
 #pragma unroll
@@ -827,8 +872,16 @@ template <typename T, int TY, bool FAST, bool BULK>
 static void launch_step_t(const StepArgs &a, const void *const in[5], void *const out[5], const Scalars *sc,
                           double *partials, hipStream_t s)
 {
-    hipLaunchKernelGGL((step_kernel<T, TY, FAST, BULK>), dim3(a.nblocks), dim3(TY * TZ), 0, s, a,
-                       cfields<T>(in), mfields<T>(out), sc, partials);
+    const dim3 grid(a.nblocks), block(TY * TZ);
+    if (a.limiter == LIM_MINMOD)
+        hipLaunchKernelGGL((step_kernel<T, TY, FAST, BULK, LIM_MINMOD>), grid, block, 0, s, a, cfields<T>(in),
+                           mfields<T>(out), sc, partials);
+    else if (a.limiter == LIM_VANLEER)
+        hipLaunchKernelGGL((step_kernel<T, TY, FAST, BULK, LIM_VANLEER>), grid, block, 0, s, a, cfields<T>(in),
+                           mfields<T>(out), sc, partials);
+    else
+        hipLaunchKernelGGL((step_kernel<T, TY, FAST, BULK, LIM_NONE>), grid, block, 0, s, a, cfields<T>(in),
+                           mfields<T>(out), sc, partials);
 }
 
 int launch_step(int prec, int fast, int tile_y, const StepArgs &a, const void *const in[5], void *const out[5],
