@@ -35,6 +35,7 @@ struct dg_ctx {
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     bool have_state = false;
+    bool sums_stale = true;   // the step kernel does not accumulate sums
     std::string err;
 };
 
@@ -78,18 +79,21 @@ int launch_one_step(dg_ctx *c)
                              c->cfg.dt_nan_fallback, 1, c->stream);
     if (rc) return fail(c, DG_ERR_DEVICE, "finalize launch");
     c->cur = 1 - c->cur;
+    c->sums_stale = true;
     return DG_OK;
 }
 
-int reduce_current(dg_ctx *c)
+// mode 0: dt and sums of a freshly set state; 2: sums only
+int reduce_current(dg_ctx *c, int mode)
 {
     const void *u[5];
     for (int f = 0; f < 5; ++f) u[f] = c->buf[c->cur][f];
-    if (dg::launch_reduce(c->prec, c->args, u, c->partials, c->nparts_reduce, c->stream))
+    if (dg::launch_reduce(c->prec, c->fast, c->args, u, c->partials, c->nparts_reduce, c->stream))
         return fail(c, DG_ERR_DEVICE, "reduce launch");
     if (dg::launch_finalize(c->prec, c->args, c->sc, c->partials, c->nparts_reduce, c->cfg.cfl,
-                            c->cfg.dt_nan_fallback, 0, c->stream))
+                            c->cfg.dt_nan_fallback, mode, c->stream))
         return fail(c, DG_ERR_DEVICE, "finalize launch");
+    c->sums_stale = false;
     return DG_OK;
 }
 
@@ -192,7 +196,9 @@ int dg_create(const dg_config *cfg, dg_ctx **out)
     if (cfg->limiter < 0 || cfg->limiter > 2) return DG_ERR_INVALID;
     if (cfg->precision == DG_FP64 && cfg->math == DG_MATH_FAST) return DG_ERR_INVALID;
     const size_t n3 = (size_t)cfg->n[0] * cfg->n[1] * cfg->n[2];
-    if (n3 > ((size_t)1 << 34)) return DG_ERR_INVALID;
+    // the kernels address a field with 32-bit byte offsets
+    const size_t el_bytes = cfg->precision == DG_FP64 ? sizeof(double) : sizeof(float);
+    if (n3 * el_bytes >= ((size_t)1 << 32)) return DG_ERR_INVALID;
 
     dg_ctx *c = new (std::nothrow) dg_ctx();
     if (!c) return DG_ERR_ALLOC;
@@ -305,7 +311,7 @@ int dg_set_state(dg_ctx *c, const void *const f[5], int kind)
         for (int q = 0; q < 5; ++q) u[q] = c->buf[c->cur][q];
         if (dg::launch_prim_to_cons(c->prec, c->args, u, c->stream)) return fail(c, DG_ERR_DEVICE, "prim_to_cons");
     }
-    int rc = reduce_current(c);
+    int rc = reduce_current(c, 0);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_state = true;
@@ -429,8 +435,9 @@ int dg_conservation_sums(dg_ctx *c, double out[5])
 {
     if (!c || !out) return DG_ERR_INVALID;
     if (!c->have_state) return fail(c, DG_ERR_STATE, "no state set");
-    int rc = read_scalars(c);
-    if (rc) return rc;
+    int rc = DG_OK;
+    if (c->sums_stale && (rc = reduce_current(c, 2))) return rc;
+    if ((rc = read_scalars(c))) return rc;
     for (int q = 0; q < 5; ++q) out[q] = c->sc_host->sums[q];
     return DG_OK;
 }

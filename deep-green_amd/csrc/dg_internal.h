@@ -11,7 +11,8 @@ enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_WALL = 2 };
 enum : int { LIM_NONE = 0, LIM_MINMOD = 1, LIM_VANLEER = 2 };
 
 // Per-workgroup partial results written by the step and reduction kernels:
-// [0] max wave-speed denominator, [1] NaN seen (0/1), [2..6] conserved sums.
+// [0] max wave-speed denominator, [1] NaN seen (0/1), [2..6] conserved sums (zero from
+// the step kernel, which does not accumulate them).
 constexpr int PARTIAL_STRIDE = 8;
 
 struct StepArgs {
@@ -35,7 +36,7 @@ struct Scalars {
     long long max_steps;
     int active;         // 1: the next step advances the state
     int nonfinite;      // the last reduction saw a NaN wave speed
-    double sums[5];     // FP64 conservation sums of the current state
+    double sums[5];     // FP64 conservation sums (of the state they were computed for)
     double maxden;      // max of sqrt(gamma p / rho) + |v| over the interior
 };
 
@@ -45,10 +46,11 @@ template <typename T> struct Fields { T *f[5]; };
 // 2 FP32_BULK; `fast` selects the reassociated arithmetic.
 int launch_step(int prec, int fast, int tile_y, const StepArgs &a, const void *const in[5],
                 void *const out[5], const Scalars *sc, double *partials, hipStream_t s);
-int launch_reduce(int prec, const StepArgs &a, const void *const u[5], double *partials,
+int launch_reduce(int prec, int fast, const StepArgs &a, const void *const u[5], double *partials,
                   int nblocks, hipStream_t s);
+// mode 0: dt and sums of a fresh state; 1: after a step; 2: sums only
 int launch_finalize(int prec, const StepArgs &a, Scalars *sc, const double *partials, int nparts,
-                    double cfl, int nan_fallback, int advance, hipStream_t s);
+                    double cfl, int nan_fallback, int mode, hipStream_t s);
 int launch_prim_to_cons(int prec, const StepArgs &a, void *const u[5], hipStream_t s);
 int launch_cons_to_prim(int prec, const StepArgs &a, const void *const u[5], void *const w[5],
                         hipStream_t s);

@@ -235,6 +235,7 @@ struct StepLDS {
     static constexpr int PS = 5 * FS;         // one plane
     T W[3 * PS];             // primitives ring [plane t mod 3][field][row][col]
     T Ym[5][TY + 1][TZ];     // -y face state of rows 0..TY
+    T Yp[5][TZ];             // +y face state of row -1
     T Yb[5][TZ];             // flux through the tile's lower y face
     T FY[5][TY][TZ];         // flux through face (row, row+1)
     T Zp[5][TY];             // +z face state of column -1
@@ -340,18 +341,21 @@ __device__ __forceinline__ void eval_cell(const StepLDS<T, TY> &L, int sm, int s
     }
 }
 
-// Face state of a cell along axis d: xtr + sign * slope * ds/2 (physics.rs:313-336).
+// Face state of a cell along axis d: xtr + sign * slope * ds/2
+// (physics.rs:313-336).  Slopes are read as g[f*stride + d*(stride==3)]:
+// stride 3 for the full slope set, 1 for a saved single-axis slope.
 template <typename T, bool FAST>
-__device__ __forceinline__ void face_state(const T xtr[5], const T g[15], int d, bool plus, const Consts<T> &C,
-                                           T s[5])
+__device__ __forceinline__ void face_state(const T xtr[5], const T *g, int d, bool plus, const Consts<T> &C,
+                                           T s[5], int stride = 3)
 {
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
+        const T gv = stride == 3 ? g[f * 3 + d] : g[f];
         if constexpr (!FAST) {
-            const T q = g[f * 3 + d] * C.half_ds;
+            const T q = gv * C.half_ds;
             s[f] = plus ? xtr[f] + q : xtr[f] - q;
         } else {
-            s[f] = dfma<T>(plus ? C.half_ds : -C.half_ds, g[f * 3 + d], xtr[f]);
+            s[f] = dfma<T>(plus ? C.half_ds : -C.half_ds, gv, xtr[f]);
         }
     }
 }
@@ -430,12 +434,12 @@ __device__ __forceinline__ void load_plane(StepLDS<T, TY> &L, int slot, int xg, 
 {
     constexpr int FS = StepLDS<T, TY>::FS, PS = StepLDS<T, TY>::PS;
     const AxisMap mx = map_axis(xg, A.n[0], A.bc[0], A.bc[1]);
-    const size_t plane = (size_t)mx.idx * A.n[1] * A.n[2];
+    const uint32_t plane = (uint32_t)mx.idx * (uint32_t)A.n[1] * (uint32_t)A.n[2];
     T *Ws = L.W + slot * PS;
 #pragma unroll
     for (int q = 0; q < LoadPlan<TY>::NS; ++q) {
         if (P.lds[q] < 0) continue;
-        const size_t idx = plane + (size_t)P.off[q];
+        const uint32_t idx = plane + (uint32_t)P.off[q];
         T u[5], w[5];
 #pragma unroll
         for (int f = 0; f < 5; ++f) u[f] = U.f[f][idx];
@@ -511,8 +515,8 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
     const int xend = min(x0 + A.lx, nx);
     const int yc = y0 + ty, zc = z0 + tz;
     const bool valid = yc < ny && zc < nz;
-    const size_t plane_sz = (size_t)ny * nz;
-    const size_t col = (size_t)yc * nz + zc;
+    const uint32_t plane_sz = (uint32_t)ny * (uint32_t)nz;   // n^3 * sizeof(T) < 4 GiB (checked at create)
+    const uint32_t col = (uint32_t)yc * (uint32_t)nz + (uint32_t)zc;
 
     Consts<T> C;
     C.ds = T(A.ds);
@@ -528,24 +532,14 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
     C.half_dt = FAST ? T(0.5) * dt * C.inv_two_ds : T(0.5) * dt;
     C.dtds = FAST ? dt * C.ds * C.ds * T(0.5) : dt * C.ds;
 
-    double sums[5] = {0, 0, 0, 0, 0};
-    double dmaxden = 0.0;
-    T maxden = T(0);
-    int nanf = 0;
-
     if (!sc->active) {
         // inactive step of a captured run: carry the state through unchanged
         if (valid)
             for (int x = x0; x < xend; ++x)
-                for (int f = 0; f < 5; ++f) Vout.f[f][x * plane_sz + col] = U.f[f][x * plane_sz + col];
+                for (int f = 0; f < 5; ++f)
+                    Vout.f[f][(uint32_t)x * plane_sz + col] = U.f[f][(uint32_t)x * plane_sz + col];
         return;
     }
-
-    // per-thread carried values (own column)
-    T xprev[5], gxprev[5];           // xtr and x-slope of plane t-1
-    T fxlo[5], fyu[5], fyl[5], fzu[5], fzl[5];  // flux parts of plane t-1's cell
-#pragma unroll
-    for (int f = 0; f < 5; ++f) { xprev[f] = gxprev[f] = fxlo[f] = fyu[f] = fyl[f] = fzu[f] = fzl[f] = T(0); }
 
     // plane-independent addressing, evaluated once
     LoadPlan<TY> plan;
@@ -555,11 +549,26 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
     if (ty == 0) halo = make_desc(-1, tz, y0, z0, A, TY);
     else if (ty == 1) halo = make_desc(TY, tz, y0, z0, A, TY);
     else if (ty == 2 && tz < 2 * TY) halo = make_desc(tz < TY ? tz : tz - TY, tz < TY ? -1 : TZ, y0, z0, A, TY);
+    const bool zgrav = A.gravity != 0;
+    const T hz = zgrav ? grav_h<T>(zc, nz, C.ds) : T(0);
+    // periodic: the reference copies the lower z face from face (n-1, n)
+    // (boundaries.rs:174-178 on fluxes), gravity height included
+    const T hzb = zgrav ? grav_h<T>((z0 == 0 && A.bc[4] == BC_PERIODIC) ? nz - 1 : z0 - 1, nz, C.ds) : T(0);
+    const bool ylo_out = yc == 0 && A.bc[2] == BC_OUTFLOW;
+    const bool zlo_out = zc == 0 && A.bc[4] == BC_OUTFLOW;
+
+    // carried values of the own column
+    T xprev[5], gxprev[5];                      // xtr and x-slope of plane t-1
+    T fxlo[5], fyu[5], fyl[5], fzu[5], fzl[5];  // flux parts of plane t-1's cell
+#pragma unroll
+    for (int f = 0; f < 5; ++f) { xprev[f] = gxprev[f] = fxlo[f] = fyu[f] = fyl[f] = fzu[f] = fzl[f] = T(0); }
+    T maxden = T(0);
+    double dmaxden = 0.0;
+    int nanf = 0;
 
     load_plane<T, TY, FAST>(L, ring(x0 - 2), x0 - 2, plan, A, U, C);
     load_plane<T, TY, FAST>(L, ring(x0 - 1), x0 - 1, plan, A, U, C);
 
-    const bool zgrav = A.gravity != 0;
     for (int t = x0 - 1; t <= xend; ++t) {
         const bool full = t >= x0 && t < xend;
         const int sm = ring(t - 1), s0 = ring(t), sp = ring(t + 1);
@@ -567,7 +576,28 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
         load_plane<T, TY, FAST>(L, sp, t + 1, plan, A, U, C);
         __syncthreads();
 
-        // ---- B: extrapolated state and slopes of the own cell
+        // ---- B: halo face states (waves 0-2), then the own cell
+        if (full && (ty < 2 || (ty == 2 && tz < 2 * TY))) {
+            T hx[5], hg[15], hs[5];
+            eval_cell<T, TY, FAST, LIM>(L, sm, s0, sp, halo, C, hx, hg);
+            if (ty == 0) {                     // row -1: +y state
+                face_state<T, FAST>(hx, hg, 1, true, C, hs);
+#pragma unroll
+                for (int f = 0; f < 5; ++f) L.Yp[f][tz] = hs[f];
+            } else if (ty == 1) {              // row TY: -y state
+                face_state<T, FAST>(hx, hg, 1, false, C, hs);
+#pragma unroll
+                for (int f = 0; f < 5; ++f) L.Ym[f][TY][tz] = hs[f];
+            } else if (tz < TY) {              // column -1: +z state
+                face_state<T, FAST>(hx, hg, 2, true, C, hs);
+#pragma unroll
+                for (int f = 0; f < 5; ++f) L.Zp[f][tz] = hs[f];
+            } else {                           // column 64: -z state
+                face_state<T, FAST>(hx, hg, 2, false, C, hs);
+#pragma unroll
+                for (int f = 0; f < 5; ++f) L.Zm[f][tz - TY] = hs[f];
+            }
+        }
         T xtr[5], g[15];
         eval_cell<T, TY, FAST, LIM>(L, sm, s0, sp, own, C, xtr, g);
         T szm[5];
@@ -581,62 +611,28 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
 #pragma unroll
                 for (int f = 0; f < 5; ++f) L.Z0[f][ty] = szm[f];
             }
-            // halo states
-            if (ty == 0) {
-                T hx[5], hg[15], hs[5], F[5];
-                eval_cell<T, TY, FAST, LIM>(L, sm, s0, sp, halo, C, hx, hg);
-                face_state<T, FAST>(hx, hg, 1, true, C, hs);
-                face_flux<T, FAST>(sym, hs, 1, C, false, T(0), F);   // face (-1, 0)
-#pragma unroll
-                for (int f = 0; f < 5; ++f) L.Yb[f][tz] = F[f];
-            } else if (ty == 1) {
-                T hx[5], hg[15], hs[5];
-                eval_cell<T, TY, FAST, LIM>(L, sm, s0, sp, halo, C, hx, hg);
-                face_state<T, FAST>(hx, hg, 1, false, C, hs);
-#pragma unroll
-                for (int f = 0; f < 5; ++f) L.Ym[f][TY][tz] = hs[f];
-            } else if (ty == 2 && tz < 2 * TY) {
-                const bool lowz = tz < TY;
-                const int row = lowz ? tz : tz - TY;
-                T hx[5], hg[15], hs[5];
-                eval_cell<T, TY, FAST, LIM>(L, sm, s0, sp, halo, C, hx, hg);
-                face_state<T, FAST>(hx, hg, 2, lowz, C, hs);
-                if (lowz) {
-#pragma unroll
-                    for (int f = 0; f < 5; ++f) L.Zp[f][row] = hs[f];
-                } else {
-#pragma unroll
-                    for (int f = 0; f < 5; ++f) L.Zm[f][row] = hs[f];
-                }
-            }
         }
         __syncthreads();
 
-        // ---- C: fluxes through the own cell's faces
+        // ---- C: fluxes through the own cell's +x (of plane t-1), +y, +z faces
         T fxu[5] = {T(0), T(0), T(0), T(0), T(0)};  // face (t-1, t) along x
         if (t >= x0) {
             T r[5], l[5];
-#pragma unroll
-            for (int f = 0; f < 5; ++f) {
-                const T qp = gxprev[f] * C.half_ds, qc = g[f * 3 + 0] * C.half_ds;
-                r[f] = xprev[f] + qp;
-                l[f] = xtr[f] - qc;
-            }
+            face_state<T, FAST>(xprev, gxprev, 0, true, C, r, 1);
+            face_state<T, FAST>(xtr, g, 0, false, C, l);
             if (t == 0 && A.bc[0] == BC_WALL) {      // ghost plane -1 mirrors plane 0
-#pragma unroll
-                for (int f = 0; f < 5; ++f) r[f] = (f == 1 ? -xtr[f] : xtr[f]) + gxprev[f] * C.half_ds;
+                T m[5] = {xtr[0], -xtr[1], xtr[2], xtr[3], xtr[4]};
+                face_state<T, FAST>(m, gxprev, 0, true, C, r, 1);
             }
-            if (t == nx && A.bc[1] == BC_OUTFLOW) {  // ghost nx copies plane nx-1
-#pragma unroll
-                for (int f = 0; f < 5; ++f) l[f] = xprev[f] - gxprev[f] * C.half_ds;
-            }
+            if (t == nx && A.bc[1] == BC_OUTFLOW)    // ghost nx copies plane nx-1
+                face_state<T, FAST>(xprev, gxprev, 0, false, C, l, 1);
             if (t == nx && A.bc[1] == BC_WALL) {
-#pragma unroll
-                for (int f = 0; f < 5; ++f) l[f] = (f == 1 ? -xprev[f] : xprev[f]) - g[f * 3 + 0] * C.half_ds;
+                T m[5] = {xprev[0], -xprev[1], xprev[2], xprev[3], xprev[4]};
+                face_state<T, FAST>(m, g, 0, false, C, l);
             }
             face_flux<T, FAST>(l, r, 0, C, false, T(0), fxu);
         }
-        T fyu_n[5], fzu_n[5], fyl_n[5], fzl_n[5];
+        T fyu_n[5], fzu_n[5];
         if (full) {
             T r[5], l[5];
             face_state<T, FAST>(xtr, g, 1, true, C, r);
@@ -651,38 +647,37 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
                 const T nb = __shfl_down(szm[f], 1);
                 l[f] = tz == TZ - 1 ? L.Zm[f][ty] : nb;
             }
-            const T h = zgrav ? grav_h<T>(zc, nz, C.ds) : T(0);
-            face_flux<T, FAST>(l, r, 2, C, zgrav, h, fzu_n);
-            if (ty == 2 && tz < TY) {                // the tile's lower z faces
+            face_flux<T, FAST>(l, r, 2, C, zgrav, hz, fzu_n);
+            if (ty == 0) {                           // the tile's lower y faces
+                T rb[5], lb[5], Fb[5];
+#pragma unroll
+                for (int f = 0; f < 5; ++f) { rb[f] = L.Yp[f][tz]; lb[f] = L.Ym[f][0][tz]; }
+                face_flux<T, FAST>(lb, rb, 1, C, false, T(0), Fb);
+#pragma unroll
+                for (int f = 0; f < 5; ++f) L.Yb[f][tz] = Fb[f];
+            } else if (ty == 2 && tz < TY) {         // the tile's lower z faces
                 T rb[5], lb[5], Fb[5];
 #pragma unroll
                 for (int f = 0; f < 5; ++f) { rb[f] = L.Zp[f][tz]; lb[f] = L.Z0[f][tz]; }
-                // periodic: the reference copies this face from face (n-1, n)
-                // (boundaries.rs:174-178 on fluxes), gravity height included
-                const int zh = (z0 == 0 && A.bc[4] == BC_PERIODIC) ? nz - 1 : z0 - 1;
-                const T hb = zgrav ? grav_h<T>(zh, nz, C.ds) : T(0);
-                face_flux<T, FAST>(lb, rb, 2, C, zgrav, hb, Fb);
+                face_flux<T, FAST>(lb, rb, 2, C, zgrav, hzb, Fb);
 #pragma unroll
                 for (int f = 0; f < 5; ++f) L.FZb[f][tz] = Fb[f];
             }
         }
-        if (full) __syncthreads();
+#pragma unroll
+        for (int f = 0; f < 5; ++f) { xprev[f] = xtr[f]; gxprev[f] = g[f * 3 + 0]; }
+        __syncthreads();
 
-        // ---- D: gather lower y/z fluxes, finish cell t-1, carry plane t
+        // ---- D: lower y/z fluxes, finish cell t-1, carry plane t
+        T fyl_n[5], fzl_n[5];
         if (full) {
 #pragma unroll
             for (int f = 0; f < 5; ++f) {
                 fyl_n[f] = ty == 0 ? L.Yb[f][tz] : L.FY[f][ty - 1][tz];
                 const T up = __shfl_up(fzu_n[f], 1);
                 fzl_n[f] = tz == 0 ? L.FZb[f][ty] : up;
-            }
-            if (yc == 0 && A.bc[2] == BC_OUTFLOW) {
-#pragma unroll
-                for (int f = 0; f < 5; ++f) fyl_n[f] = fyu_n[f];
-            }
-            if (zc == 0 && A.bc[4] == BC_OUTFLOW) {
-#pragma unroll
-                for (int f = 0; f < 5; ++f) fzl_n[f] = fzu_n[f];
+                if (ylo_out) fyl_n[f] = fyu_n[f];    // boundaries.rs:137-141 on fluxes
+                if (zlo_out) fzl_n[f] = fzu_n[f];
             }
         }
         const int xc = t - 1;
@@ -691,7 +686,7 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
 #pragma unroll
                 for (int f = 0; f < 5; ++f) fxlo[f] = fxu[f];
             }
-            const size_t idx = (size_t)xc * plane_sz + col;
+            const uint32_t idx = (uint32_t)xc * plane_sz + col;
             T un[5];
 #pragma unroll
             for (int f = 0; f < 5; ++f) {
@@ -699,9 +694,8 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
                 T acc;
                 if constexpr (!FAST) acc = ((((fxu[f] - fxlo[f]) + fyu[f]) - fyl[f]) + fzu[f]) - fzl[f];
                 else acc = (fxu[f] - fxlo[f]) + fyu[f];   // fyu holds the y/z partial sum
-                un[f] = u - C.dtds * acc;
+                un[f] = FAST ? dfma<T>(-C.dtds, acc, u) : u - C.dtds * acc;
                 Vout.f[f][idx] = un[f];
-                sums[f] += (double)un[f];
             }
             T w[5];
             to_prim<T, FAST>(un, C.V, C.invV, C.gm1, w);
@@ -726,17 +720,15 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
                 }
             }
         }
-#pragma unroll
-        for (int f = 0; f < 5; ++f) { xprev[f] = xtr[f]; gxprev[f] = g[f * 3 + 0]; }
-        if (!full) __syncthreads();   // ring slot sm is overwritten next iteration
     }
 
     if constexpr (!BULK) dmaxden = (double)maxden;
-    block_partials<NT>(dmaxden, nanf, sums, red, partials, blockIdx.x, tid);
+    const double nosums[5] = {0, 0, 0, 0, 0};
+    block_partials<NT>(dmaxden, nanf, nosums, red, partials, blockIdx.x, tid);
 }
 
 // --------------------------------------------------------- small kernels --
-template <typename T, bool BULK>
+template <typename T, bool BULK, bool FAST>
 __global__ void __launch_bounds__(256)
 reduce_kernel(const StepArgs A, const Fields<const T> U, double *__restrict__ partials)
 {
@@ -751,14 +743,14 @@ reduce_kernel(const StepArgs A, const Fields<const T> U, double *__restrict__ pa
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
         T u[5], w[5];
         for (int f = 0; f < 5; ++f) { u[f] = U.f[f][i]; sums[f] += (double)u[f]; }
-        to_prim<T, false>(u, V, T(1) / V, gm1, w);
+        to_prim<T, FAST>(u, V, T(1) / V, gm1, w);   // same math as the step epilogue
         if constexpr (BULK) {
             double wd[5];
             for (int f = 0; f < 5; ++f) wd[f] = (double)w[f];
             const double den = wave_den<double, false>(wd, (double)gamma);
             if (den != den) nanf = 1; else dmaxden = fmax(dmaxden, den);
         } else {
-            const T den = wave_den<T, false>(w, gamma);
+            const T den = wave_den<T, FAST>(w, gamma);
             if (den != den) nanf = 1; else maxden = dmax<T>(maxden, den);
         }
     }
@@ -769,10 +761,12 @@ reduce_kernel(const StepArgs A, const Fields<const T> U, double *__restrict__ pa
 // dt from the partials (calc_timestep, physics.rs:177-196: CFL * min over
 // cells of ds / den == CFL * ds / max den), conservation sums, and the run
 // clock (main.rs:296-298).  One workgroup; fixed reduction order.
+// mode 0: dt and sums of a fresh state; 1: after a step (advance the clock,
+// new dt); 2: sums only.
 template <typename T, bool BULK>
 __global__ void __launch_bounds__(1024)
 finalize_kernel(const StepArgs A, Scalars *__restrict__ sc, const double *__restrict__ partials, int np,
-                double cfl, int nan_fallback, int advance)
+                double cfl, int nan_fallback, int mode)
 {
     __shared__ double red[16 * 8];
     const int tid = threadIdx.x;
@@ -800,7 +794,11 @@ finalize_kernel(const StepArgs A, Scalars *__restrict__ sc, const double *__rest
         o[1] = fmax(o[1], red[w * 8 + 1]);
         for (int q = 2; q < 7; ++q) o[q] += red[w * 8 + q];
     }
-    if (advance) {
+    if (mode == 2) {
+        for (int q = 0; q < 5; ++q) sc->sums[q] = o[2 + q];
+        return;
+    }
+    if (mode == 1) {
         if (!sc->active) return;
         if constexpr (!BULK) sc->t = (double)(T(sc->t) + T(sc->dt));
         else sc->t = sc->t + sc->dt;
@@ -818,7 +816,8 @@ finalize_kernel(const StepArgs A, Scalars *__restrict__ sc, const double *__rest
     sc->dt = dt;
     sc->nonfinite = nan ? 1 : 0;
     sc->maxden = o[0];
-    for (int q = 0; q < 5; ++q) sc->sums[q] = o[2 + q];
+    if (mode == 0)
+        for (int q = 0; q < 5; ++q) sc->sums[q] = o[2 + q];
     sc->active = !((sc->t > sc->t_end) || (sc->steps >= sc->max_steps));
 }
 
@@ -913,7 +912,7 @@ int step_lds_bytes(int prec, int tile_y)
 {
     const size_t el = prec == 0 ? sizeof(double) : sizeof(float);
     const size_t ty = (size_t)tile_y;
-    const size_t cells = 3 * 5 * (ty + 4) * RZ + 5 * (ty + 1) * TZ + 5 * TZ + 5 * ty * TZ + 4 * 5 * ty;
+    const size_t cells = 3 * 5 * (ty + 4) * RZ + 5 * (ty + 1) * TZ + 2 * 5 * TZ + 5 * ty * TZ + 4 * 5 * ty;
     return (int)(cells * el + (ty * TZ / 64) * 8 * sizeof(double));
 }
 
@@ -926,27 +925,32 @@ int reduce_blocks(const StepArgs &a)
     return (int)nb;
 }
 
-int launch_reduce(int prec, const StepArgs &a, const void *const u[5], double *partials, int nblocks,
+int launch_reduce(int prec, int fast, const StepArgs &a, const void *const u[5], double *partials, int nblocks,
                   hipStream_t s)
 {
+    const dim3 g(nblocks), b(256);
     if (prec == 0)
-        hipLaunchKernelGGL((reduce_kernel<double, false>), dim3(nblocks), dim3(256), 0, s, a, cfields<double>(u), partials);
+        hipLaunchKernelGGL((reduce_kernel<double, false, false>), g, b, 0, s, a, cfields<double>(u), partials);
+    else if (prec == 1 && !fast)
+        hipLaunchKernelGGL((reduce_kernel<float, false, false>), g, b, 0, s, a, cfields<float>(u), partials);
     else if (prec == 1)
-        hipLaunchKernelGGL((reduce_kernel<float, false>), dim3(nblocks), dim3(256), 0, s, a, cfields<float>(u), partials);
+        hipLaunchKernelGGL((reduce_kernel<float, false, true>), g, b, 0, s, a, cfields<float>(u), partials);
+    else if (!fast)
+        hipLaunchKernelGGL((reduce_kernel<float, true, false>), g, b, 0, s, a, cfields<float>(u), partials);
     else
-        hipLaunchKernelGGL((reduce_kernel<float, true>), dim3(nblocks), dim3(256), 0, s, a, cfields<float>(u), partials);
+        hipLaunchKernelGGL((reduce_kernel<float, true, true>), g, b, 0, s, a, cfields<float>(u), partials);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int launch_finalize(int prec, const StepArgs &a, Scalars *sc, const double *partials, int nparts, double cfl,
-                    int nan_fallback, int advance, hipStream_t s)
+                    int nan_fallback, int mode, hipStream_t s)
 {
     if (prec == 0)
-        hipLaunchKernelGGL((finalize_kernel<double, false>), dim3(1), dim3(1024), 0, s, a, sc, partials, nparts, cfl, nan_fallback, advance);
+        hipLaunchKernelGGL((finalize_kernel<double, false>), dim3(1), dim3(1024), 0, s, a, sc, partials, nparts, cfl, nan_fallback, mode);
     else if (prec == 1)
-        hipLaunchKernelGGL((finalize_kernel<float, false>), dim3(1), dim3(1024), 0, s, a, sc, partials, nparts, cfl, nan_fallback, advance);
+        hipLaunchKernelGGL((finalize_kernel<float, false>), dim3(1), dim3(1024), 0, s, a, sc, partials, nparts, cfl, nan_fallback, mode);
     else
-        hipLaunchKernelGGL((finalize_kernel<float, true>), dim3(1), dim3(1024), 0, s, a, sc, partials, nparts, cfl, nan_fallback, advance);
+        hipLaunchKernelGGL((finalize_kernel<float, true>), dim3(1), dim3(1024), 0, s, a, sc, partials, nparts, cfl, nan_fallback, mode);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
