@@ -59,6 +59,44 @@ template <> __device__ __forceinline__ double dabs<double>(double x) { return fa
 template <typename T> __device__ __forceinline__ T drcp(T x);
 template <> __device__ __forceinline__ float drcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template <> __device__ __forceinline__ double drcp<double>(double x) { return 1.0 / x; }
+template <typename T> __device__ __forceinline__ T dsqrt_fast(T x);
+template <> __device__ __forceinline__ float dsqrt_fast<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <> __device__ __forceinline__ double dsqrt_fast<double>(double x) { return sqrt(x); }
+
+// Whole-wavefront lane shifts (DPP wave_shl:1 / wave_shr:1): lane i receives
+// lane i+1 (shl) or lane i-1 (shr); the lane without a source gets 0.
+__device__ __forceinline__ int dpp_from_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_from_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+template <typename T> __device__ __forceinline__ T from_next_lane(T v);
+template <typename T> __device__ __forceinline__ T from_prev_lane(T v);
+template <> __device__ __forceinline__ float from_next_lane<float>(float v)
+{ return __builtin_bit_cast(float, dpp_from_next(__builtin_bit_cast(int, v))); }
+template <> __device__ __forceinline__ float from_prev_lane<float>(float v)
+{ return __builtin_bit_cast(float, dpp_from_prev(__builtin_bit_cast(int, v))); }
+template <> __device__ __forceinline__ double from_next_lane<double>(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = dpp_from_next((int)(b & 0xffffffffLL)), hi = dpp_from_next((int)(b >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <> __device__ __forceinline__ double from_prev_lane<double>(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = dpp_from_prev((int)(b & 0xffffffffLL)), hi = dpp_from_prev((int)(b >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Element idx of a field addressed with a 32-bit byte offset from the field
+// base (a field is < 4 GiB; the base stays in scalar registers).
+template <typename T> __device__ __forceinline__ const T &at(const T *base, uint32_t idx)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <typename T> __device__ __forceinline__ T &at(T *base, uint32_t idx)
+{
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+
 template <typename T> __device__ __forceinline__ T dfma(T a, T b, T c);
 template <> __device__ __forceinline__ float dfma<float>(float a, float b, float c) { return fmaf(a, b, c); }
 template <> __device__ __forceinline__ double dfma<double>(double a, double b, double c) { return fma(a, b, c); }
@@ -129,7 +167,7 @@ __device__ __forceinline__ T wave_den(const T w[5], T gamma)
         return dsqrt<T>(gamma * w[4] / w[0]) + dsqrt<T>(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
     } else {
         const T v2 = dfma<T>(w[3], w[3], dfma<T>(w[2], w[2], w[1] * w[1]));
-        return dsqrt<T>(gamma * w[4] * drcp<T>(w[0])) + dsqrt<T>(v2);
+        return dsqrt_fast<T>(gamma * w[4] * drcp<T>(w[0])) + dsqrt_fast<T>(v2);
     }
 }
 
@@ -148,7 +186,7 @@ __device__ __forceinline__ T limited(T a, T b, int lim)
 // cell on the - side, both (rho, v_normal, v_t1, v_t2, p); F returns the
 // rotated fluxes multiplied by ds (FAST: not multiplied by ds).
 template <typename T, bool FAST>
-__device__ __forceinline__ void rusanov(const T l[5], const T r[5], T F[5], T gamma, T gm1, T ds,
+__device__ __forceinline__ void rusanov(const T l[5], const T r[5], T F[5], T gamma, T gm1, T igm1, T ds,
                                         bool grav, T G, T h)
 {
     const T rl = l[0], vxl = l[1], vyl = l[2], vzl = l[3], pl = l[4];
@@ -184,7 +222,6 @@ __device__ __forceinline__ void rusanov(const T l[5], const T r[5], T F[5], T ga
         // caller folds the factor 1/2 and ds into the update).
         const T mxl = rl * vxl, myl = rl * vyl, mzl = rl * vzl;
         const T mxr = rr * vxr, myr = rr * vyr, mzr = rr * vzr;
-        const T igm1 = T(1) / gm1;
         const T el = dfma<T>(pl, igm1, T(0.5) * rl * dfma<T>(vzl, vzl, dfma<T>(vyl, vyl, vxl * vxl)));
         const T er = dfma<T>(pr, igm1, T(0.5) * rr * dfma<T>(vzr, vzr, dfma<T>(vyr, vyr, vxr * vxr)));
         const T Sr = rl + rr, Sx = mxl + mxr, Sy = myl + myr, Sz = mzl + mzr, Se = el + er;
@@ -202,8 +239,8 @@ __device__ __forceinline__ void rusanov(const T l[5], const T r[5], T F[5], T ga
             f1 = dfma<T>(Sr, gh, f1);
             f4 = dfma<T>(T(0.5) * Sr * (vxl + vxr), gh, f4);
         }
-        const T al = dsqrt<T>(gamma * pl * drcp<T>(rl)) + dabs<T>(vxl);
-        const T ar = dsqrt<T>(gamma * pr * drcp<T>(rr)) + dabs<T>(vxr);
+        const T al = dsqrt_fast<T>(gamma * pl * drcp<T>(rl)) + dabs<T>(vxl);
+        const T ar = dsqrt_fast<T>(gamma * pr * drcp<T>(rr)) + dabs<T>(vxr);
         const T c = dmax<T>(al, ar);
         F[0] = dfma<T>(-c, rl - rr, f0);
         F[1] = dfma<T>(-c, mxl - mxr, f1);
@@ -225,7 +262,7 @@ __device__ __forceinline__ int rot_field(int axis, int slot)
 
 template <typename T>
 struct Consts {
-    T ds, gamma, gm1, V, invV, two_ds, inv_two_ds, half_ds, half_dt, dtds, G;
+    T ds, gamma, gm1, igm1, V, invV, two_ds, inv_two_ds, half_ds, half_dt, dtds, G;
 };
 
 // LDS layout of the step kernel.
@@ -368,7 +405,7 @@ __device__ __forceinline__ void face_flux(const T l[5], const T r[5], int axis, 
     T lr[5], rr[5], F[5];
 #pragma unroll
     for (int s = 0; s < 5; ++s) { lr[s] = l[rot_field(axis, s)]; rr[s] = r[rot_field(axis, s)]; }
-    rusanov<T, FAST>(lr, rr, F, C.gamma, C.gm1, C.ds, grav, C.G, h);
+    rusanov<T, FAST>(lr, rr, F, C.gamma, C.gm1, C.igm1, C.ds, grav, C.G, h);
 #pragma unroll
     for (int s = 0; s < 5; ++s) out[rot_field(axis, s)] = F[s];
 }
@@ -442,7 +479,7 @@ __device__ __forceinline__ void load_plane(StepLDS<T, TY> &L, int slot, int xg, 
         const uint32_t idx = plane + (uint32_t)P.off[q];
         T u[5], w[5];
 #pragma unroll
-        for (int f = 0; f < 5; ++f) u[f] = U.f[f][idx];
+        for (int f = 0; f < 5; ++f) u[f] = at(U.f[f], idx);
         to_prim<T, FAST>(u, C.V, C.invV, C.gm1, w);
         if (mx.flip) w[1] = -w[1];
         if (P.flip[q] & 2) w[2] = -w[2];
@@ -522,6 +559,7 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
     C.ds = T(A.ds);
     C.gamma = T(A.gamma);
     C.gm1 = C.gamma - T(1.0);
+    C.igm1 = T(1) / C.gm1;
     C.V = C.ds * C.ds * C.ds;
     C.invV = T(1) / C.V;
     C.two_ds = T(2.0) * C.ds;
@@ -644,7 +682,7 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
             face_state<T, FAST>(xtr, g, 2, true, C, r);
 #pragma unroll
             for (int f = 0; f < 5; ++f) {
-                const T nb = __shfl_down(szm[f], 1);
+                const T nb = from_next_lane<T>(szm[f]);
                 l[f] = tz == TZ - 1 ? L.Zm[f][ty] : nb;
             }
             face_flux<T, FAST>(l, r, 2, C, zgrav, hz, fzu_n);
@@ -674,7 +712,7 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
 #pragma unroll
             for (int f = 0; f < 5; ++f) {
                 fyl_n[f] = ty == 0 ? L.Yb[f][tz] : L.FY[f][ty - 1][tz];
-                const T up = __shfl_up(fzu_n[f], 1);
+                const T up = from_prev_lane<T>(fzu_n[f]);
                 fzl_n[f] = tz == 0 ? L.FZb[f][ty] : up;
                 if (ylo_out) fyl_n[f] = fyu_n[f];    // boundaries.rs:137-141 on fluxes
                 if (zlo_out) fzl_n[f] = fzu_n[f];
@@ -690,12 +728,12 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
             T un[5];
 #pragma unroll
             for (int f = 0; f < 5; ++f) {
-                const T u = U.f[f][idx];
+                const T u = at(U.f[f], idx);
                 T acc;
                 if constexpr (!FAST) acc = ((((fxu[f] - fxlo[f]) + fyu[f]) - fyl[f]) + fzu[f]) - fzl[f];
                 else acc = (fxu[f] - fxlo[f]) + fyu[f];   // fyu holds the y/z partial sum
                 un[f] = FAST ? dfma<T>(-C.dtds, acc, u) : u - C.dtds * acc;
-                Vout.f[f][idx] = un[f];
+                at(Vout.f[f], idx) = un[f];
             }
             T w[5];
             to_prim<T, FAST>(un, C.V, C.invV, C.gm1, w);

@@ -49,6 +49,18 @@ template <typename T> T exch(T v, int src_lane_delta, int mode)
     sync();
     return out;
 }
+inline int exch_dpp(int old, int v, int dir)
+{
+    const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid & ~63;
+    sync();
+    std::memcpy(&xbuf[tid], &v, sizeof(int));
+    sync();
+    const int src = lane + dir;
+    int out = old;
+    if (src >= 0 && src < 64) std::memcpy(&out, &xbuf[wave + src], sizeof(int));
+    sync();
+    return out;
+}
 }  // namespace emu
 
 inline void __syncthreads() { emu::sync(); }
@@ -56,6 +68,13 @@ template <typename T> inline T __shfl_down(T v, unsigned d) { return emu::exch<T
 template <typename T> inline T __shfl_up(T v, unsigned d) { return emu::exch<T>(v, (int)d, 1); }
 template <typename T> inline T __shfl_xor(T v, int m) { return emu::exch<T>(v, m, 2); }
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+inline float __builtin_amdgcn_sqrtf(float x) { return std::sqrt(x); }
+// DPP wave_shl:1 (0x130: lane i <- lane i+1) and wave_shr:1 (0x138: lane i <- lane i-1);
+// lanes without a source keep `old`.
+inline int __builtin_amdgcn_update_dpp(int old, int v, int ctrl, int, int, bool)
+{
+    return ctrl == 0x130 ? emu::exch_dpp(old, v, +1) : emu::exch_dpp(old, v, -1);
+}
 
 typedef int hipError_t;
 typedef void *hipStream_t;
