@@ -222,8 +222,8 @@ __device__ __forceinline__ void rusanov(const T l[5], const T r[5], T F[5], T ga
         // caller folds the factor 1/2 and ds into the update).
         const T mxl = rl * vxl, myl = rl * vyl, mzl = rl * vzl;
         const T mxr = rr * vxr, myr = rr * vyr, mzr = rr * vzr;
-        const T el = dfma<T>(pl, igm1, T(0.5) * rl * dfma<T>(vzl, vzl, dfma<T>(vyl, vyl, vxl * vxl)));
-        const T er = dfma<T>(pr, igm1, T(0.5) * rr * dfma<T>(vzr, vzr, dfma<T>(vyr, vyr, vxr * vxr)));
+        const T el = dfma<T>(T(0.5), dfma<T>(mzl, vzl, dfma<T>(myl, vyl, mxl * vxl)), pl * igm1);
+        const T er = dfma<T>(T(0.5), dfma<T>(mzr, vzr, dfma<T>(myr, vyr, mxr * vxr)), pr * igm1);
         const T Sr = rl + rr, Sx = mxl + mxr, Sy = myl + myr, Sz = mzl + mzr, Se = el + er;
         const T iS = drcp<T>(Sr);
         const T m2 = dfma<T>(Sz, Sz, dfma<T>(Sy, Sy, Sx * Sx));
@@ -593,7 +593,8 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
     // (boundaries.rs:174-178 on fluxes), gravity height included
     const T hzb = zgrav ? grav_h<T>((z0 == 0 && A.bc[4] == BC_PERIODIC) ? nz - 1 : z0 - 1, nz, C.ds) : T(0);
     const bool ylo_out = yc == 0 && A.bc[2] == BC_OUTFLOW;
-    const bool zlo_out = zc == 0 && A.bc[4] == BC_OUTFLOW;
+    const bool zlo_tile = z0 == 0 && A.bc[4] == BC_OUTFLOW;   // uniform
+    const bool zlo_out = zlo_tile && tz == 0;
 
     // carried values of the own column
     T xprev[5], gxprev[5];                      // xtr and x-slope of plane t-1
@@ -654,18 +655,24 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
 
         // ---- C: fluxes through the own cell's +x (of plane t-1), +y, +z faces
         T fxu[5] = {T(0), T(0), T(0), T(0), T(0)};  // face (t-1, t) along x
-        if (t >= x0) {
+        const bool xbnd = (t == 0 && A.bc[0] == BC_WALL) || (t == nx && A.bc[1] != BC_PERIODIC);
+        if (t >= x0 && !xbnd) {
             T r[5], l[5];
             face_state<T, FAST>(xprev, gxprev, 0, true, C, r, 1);
             face_state<T, FAST>(xtr, g, 0, false, C, l);
-            if (t == 0 && A.bc[0] == BC_WALL) {      // ghost plane -1 mirrors plane 0
+            face_flux<T, FAST>(l, r, 0, C, false, T(0), fxu);
+        } else if (t >= x0) {                        // a domain-boundary plane (uniform branch)
+            T r[5], l[5];
+            if (t == 0) {                            // wall: ghost plane -1 mirrors plane 0
                 T m[5] = {xtr[0], -xtr[1], xtr[2], xtr[3], xtr[4]};
                 face_state<T, FAST>(m, gxprev, 0, true, C, r, 1);
-            }
-            if (t == nx && A.bc[1] == BC_OUTFLOW)    // ghost nx copies plane nx-1
+                face_state<T, FAST>(xtr, g, 0, false, C, l);
+            } else if (A.bc[1] == BC_OUTFLOW) {      // ghost nx copies plane nx-1
+                face_state<T, FAST>(xprev, gxprev, 0, true, C, r, 1);
                 face_state<T, FAST>(xprev, gxprev, 0, false, C, l, 1);
-            if (t == nx && A.bc[1] == BC_WALL) {
+            } else {                                 // wall: ghost nx mirrors plane nx-1
                 T m[5] = {xprev[0], -xprev[1], xprev[2], xprev[3], xprev[4]};
+                face_state<T, FAST>(xprev, gxprev, 0, true, C, r, 1);
                 face_state<T, FAST>(m, g, 0, false, C, l);
             }
             face_flux<T, FAST>(l, r, 0, C, false, T(0), fxu);
@@ -686,14 +693,16 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
                 l[f] = tz == TZ - 1 ? L.Zm[f][ty] : nb;
             }
             face_flux<T, FAST>(l, r, 2, C, zgrav, hz, fzu_n);
-            if (ty == 0) {                           // the tile's lower y faces
+            // the tile's lower y faces (wave 3) and lower z faces (wave 0):
+            // waves on the SIMDs that evaluated no halo row in phase B
+            if (ty == 3) {
                 T rb[5], lb[5], Fb[5];
 #pragma unroll
                 for (int f = 0; f < 5; ++f) { rb[f] = L.Yp[f][tz]; lb[f] = L.Ym[f][0][tz]; }
                 face_flux<T, FAST>(lb, rb, 1, C, false, T(0), Fb);
 #pragma unroll
                 for (int f = 0; f < 5; ++f) L.Yb[f][tz] = Fb[f];
-            } else if (ty == 2 && tz < TY) {         // the tile's lower z faces
+            } else if (ty == 0 && tz < TY) {
                 T rb[5], lb[5], Fb[5];
 #pragma unroll
                 for (int f = 0; f < 5; ++f) { rb[f] = L.Zp[f][tz]; lb[f] = L.Z0[f][tz]; }
@@ -714,8 +723,14 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
                 fyl_n[f] = ty == 0 ? L.Yb[f][tz] : L.FY[f][ty - 1][tz];
                 const T up = from_prev_lane<T>(fzu_n[f]);
                 fzl_n[f] = tz == 0 ? L.FZb[f][ty] : up;
-                if (ylo_out) fyl_n[f] = fyu_n[f];    // boundaries.rs:137-141 on fluxes
-                if (zlo_out) fzl_n[f] = fzu_n[f];
+            }
+            if (ylo_out) {                           // boundaries.rs:137-141 on fluxes
+#pragma unroll
+                for (int f = 0; f < 5; ++f) fyl_n[f] = fyu_n[f];
+            }
+            if (zlo_tile) {                          // boundaries.rs:167-171 on fluxes
+#pragma unroll
+                for (int f = 0; f < 5; ++f) fzl_n[f] = zlo_out ? fzu_n[f] : fzl_n[f];
             }
         }
         const int xc = t - 1;
