@@ -1,23 +1,31 @@
 """Benchmark: fused compressible-Euler step on one MI355X.
 
-Workload (default): 3D Kelvin-Helmholtz shear layer, 512^3 periodic, FP32 SoA
-state -- the configuration the HBM-roofline figure is quoted on.  One "step"
-is one full reference step (primitives, dt reduction, slopes, Hancock
-half-step, Rusanov fluxes on all faces, conservative update).  Inputs are
-resident in HBM before the timed region; the timed region runs K steps from
-the library's captured HIP graph with the dt kept on the device.
+Workload (default, --workload khi): 3D Kelvin-Helmholtz shear layer, 512^3
+periodic, FP32 SoA state -- the configuration the HBM-roofline figure is
+quoted on.  One "step" is one full reference step (primitives, dt reduction,
+slopes, Hancock half-step, Rusanov fluxes on all faces, conservative update).
+Inputs are resident in HBM before the timed region; the timed region runs K
+steps from the library's captured HIP graph with the dt kept on the device.
+
+--workload ensemble: 256 cubic grids with edges drawn by seed from
+{32, 48, 64, 96, 128}, random planar Riemann problems, outflow boundaries,
+minmod slopes, FP32; one step advances every grid (one persistent step
+launch + one dt launch, replayed from a captured HIP graph); tiles from the
+autotuner's cache (tuned first if the cache has no entry).
 
 Prints one JSON line (rank 0).  Fields beyond the contract:
   roofline      dominant kernel (the fused step) against HBM: algorithmic
                 bytes per launch (read U + write U: 10 * 4 B per interior
                 cell) / mean launch time from HIP events on the library's
-                stream.
+                stream; traffic = HBM bytes per launch measured by rocprofv3
+                (FETCH_SIZE + WRITE_SIZE, copy-calibrated) where a committed
+                profile summary holds it.
   cpu_baseline  the reference CPU path on this host, on a bounded sample.
 """
 import argparse
 import json
 import os
-import subprocess
+import re
 import sys
 import time
 
@@ -29,24 +37,16 @@ sys.path.insert(0, os.path.join(ROOT, "deep-green_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
-def khi_state(n, dtype, seed=1):
-    """Shear layer along y (|y - 1/2| < 1/4: rho 2, u +0.5; else rho 1,
-    u -0.5), p 2.5, v = 0.01 sin(4 pi x) + U(-1e-3, 1e-3) from the seed,
-    cell-centred coordinates on the unit cube; ghosted C-order arrays."""
-    ds = 1.0 / n
-    c = (np.arange(n + 4) - 2 + 0.5) * ds
-    shape = (n + 4,) * 3
-    inner = (np.abs(c - 0.5) < 0.25)[None, :, None]
-    rho = np.empty(shape, dtype)
-    rho[...] = np.where(inner, 2.0, 1.0)
-    u = np.empty(shape, dtype)
-    u[...] = np.where(inner, 0.5, -0.5)
-    rng = np.random.default_rng(seed)
-    v = rng.uniform(-1e-3, 1e-3, size=shape).astype(dtype)
-    v += (0.01 * np.sin(4 * np.pi * c)).astype(dtype)[:, None, None]
-    w = np.zeros(shape, dtype)
-    p = np.full(shape, 2.5, dtype)
-    return [rho, u, v, w, p], ds
+def traffic_from_profile(name):
+    """HBM bytes per launch recorded in profiles/<name> (the line
+    'hbm_bytes_per_launch: <int>'), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            m = re.search(r"^hbm_bytes_per_launch:\s*(\d+)", f.read(), re.M)
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def cpu_baseline(n_sample=128, steps=3):
@@ -57,7 +57,8 @@ def cpu_baseline(n_sample=128, steps=3):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import dg_oracle
     cores = os.cpu_count() or 1
-    w, ds = khi_state(n_sample, np.float64)
+    from deepgreen.setups import khi
+    w, ds = khi(n_sample, np.float64)
     cells = n_sample ** 3 * steps
     if dg_oracle.ref_binary("sp", "cfg"):
         _, _, wall = dg_oracle.run_reference_fortran(w, ds, [0] * 6, steps, precision="sp", params="cfg",
@@ -77,12 +78,117 @@ def cpu_baseline(n_sample=128, steps=3):
             "sample": f"KHI {n_sample}^3, {steps} steps, {what}, {wall:.2f} s"}
 
 
+def bench_khi(args):
+    import deepgreen as dg
+    from deepgreen.setups import khi
+
+    n = args.n
+    dtype = np.float64 if args.precision == "fp64" else np.float32
+    w, ds = khi(n, dtype)
+    sim = dg.Simulation(n, n, n, ds, [dg.PERIODIC] * 6, gamma=1.4, cfl=0.4, g=0.0, precision=args.precision,
+                        math=args.math, tile_y=args.tile_y)
+    solver = dg.Solver(sim)
+    solver.set_primitives(w)
+    del w
+    info = solver.kernel_info()
+    print(f"[bench] KHI {n}^3 {args.precision}/{args.math} tile {info}", file=sys.stderr, flush=True)
+
+    solver.run(args.warmup)
+    # timed region: K steps from the captured graph, bracketed by device syncs
+    solver.next_dt()
+    t0 = time.perf_counter()
+    solver.run(args.steps)
+    solver.next_dt()  # synchronizes the stream
+    elapsed = time.perf_counter() - t0
+    cells = n ** 3
+    gcells = cells * args.steps / elapsed / 1e9
+    print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    # per-kernel device times (HIP events on the library's stream)
+    nk = min(args.steps, 20)
+    _, ms_step, ms_red = solver.time_steps(nk)
+    el = 8 if args.precision == "fp64" else 4
+    alg_bytes = 10 * el * cells
+    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
+    finite = bool(np.isfinite(solver.conservation_sums()).all())
+    solver.close()
+    workload = f"khi{n}_{args.precision}_{args.math}"
+    traffic = traffic_from_profile(f"step_kernel_{workload}_hbm.md")
+    return {
+        "s_per_step": elapsed / args.steps, "value": gcells, "dtype": "f64" if args.precision == "fp64" else "f32",
+        "config": {"workload": f"khi{n}_{args.precision}", "cells": cells, "periodic": True,
+                   "gamma": 1.4, "cfl": 0.4, "math": args.math, "limiter": "none",
+                   "pct_hbm_peak": round(100.0 * gcells * 1e9 * 10 * el / (HBM_PEAK_GBS * 1e9), 2),
+                   "tile": info, "state_finite": finite},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "step_kernel", "kernel_ms": round(ms_step, 4),
+                     "reduce_kernel_ms": round(ms_red, 4), "alg_bytes_per_launch": alg_bytes},
+    }
+
+
+def bench_ensemble(args):
+    import deepgreen as dg
+    from deepgreen import autotune
+    from deepgreen.setups import ensemble_members, ensemble_workload
+
+    members = ensemble_members(args.members, args.seed)
+    sizes = sorted({n for n, _ in members})
+    counts = [sum(1 for m, _ in members if m == n) for n in sizes]
+    t0 = time.perf_counter()
+    choice = autotune.lookup(sizes, math=args.math)
+    tuned_now = choice is None
+    if tuned_now:
+        choice = autotune.tune(sizes, counts, math=args.math,
+                               log=lambda s: print(s, file=sys.stderr, flush=True))
+    tune_s = time.perf_counter() - t0
+    sims, states = ensemble_workload(args.members, args.seed, math=args.math, choice=choice)
+    ens = dg.Ensemble(sims, grid=autotune.grid_for(choice["blocks_per_cu"]))
+    for i, w in states():
+        ens.set_state(i, w)
+    info = ens.info()
+    print(f"[bench] ensemble of {len(sims)} ({dict(zip(sizes, counts))}), {info}, tiles {choice['tile_y']} "
+          f"{choice['chunk_x']} blocks/CU {choice['blocks_per_cu']}", file=sys.stderr, flush=True)
+    ens.run(args.warmup)
+    ens.clock(0)
+    t0 = time.perf_counter()
+    ens.run(args.steps)
+    ens.clock(0)   # synchronizes the stream
+    elapsed = time.perf_counter() - t0
+    cells = info["cells"]
+    gcells = cells * args.steps / elapsed / 1e9
+    print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    ms_step, ms_dt = ens.time_steps(min(args.steps, 20))
+    alg_bytes = 40 * cells
+    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
+    finite = all(bool(np.isfinite(ens.conservation_sums(i)).all()) for i in range(0, len(sims), 16))
+    ens.close()
+    traffic = traffic_from_profile("ens_step_kernel_ensemble256_fp32_fast_hbm.md")
+    return {
+        "s_per_step": elapsed / args.steps, "value": gcells, "dtype": "f32",
+        "config": {"workload": f"ensemble{args.members}_fp32", "members": len(sims), "cells": cells,
+                   "sizes": dict(zip(map(str, sizes), counts)), "seed": args.seed, "boundaries": "outflow",
+                   "gamma": 1.4, "cfl": 0.4, "math": args.math, "limiter": "minmod",
+                   "pct_hbm_peak": round(100.0 * gcells * 1e9 * 40 / (HBM_PEAK_GBS * 1e9), 2),
+                   "tile_y": choice["tile_y"], "chunk_x": choice["chunk_x"],
+                   "blocks_per_cu": choice["blocks_per_cu"], "work_items": info["items"], "grid": info["grid"],
+                   "autotune_s": round(tune_s, 2), "tuned_in_run": tuned_now, "state_finite": finite},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "ens_step_kernel", "kernel_ms": round(ms_step, 4),
+                     "dt_kernel_ms": round(ms_dt, 4), "alg_bytes_per_launch": alg_bytes},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (khi: 200, ensemble: 100)")
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="khi", choices=["khi", "ensemble"])
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--math", default="fast", choices=["fast", "exact"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32_bulk", "fp64"])
     ap.add_argument("--tile-y", type=int, default=0)
@@ -105,64 +211,28 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.steps is None:
+        args.steps = 100 if args.workload == "ensemble" else 200
 
-    import deepgreen as dg
-
-    n = args.n
-    dtype = np.float64 if args.precision == "fp64" else np.float32
-    w, ds = khi_state(n, dtype)
-    sim = dg.Simulation(n, n, n, ds, [dg.PERIODIC] * 6, gamma=1.4, cfl=0.4, g=0.0, precision=args.precision,
-                        math=args.math, tile_y=args.tile_y)
-    solver = dg.Solver(sim)
-    solver.set_primitives(w)
-    del w
-    info = solver.kernel_info()
-    print(f"[bench] KHI {n}^3 {args.precision}/{args.math} tile {info}", file=sys.stderr, flush=True)
-
-    solver.run(args.warmup)
-    # timed region: K steps from the captured graph, bracketed by device syncs
-    t0 = time.perf_counter()
-    solver.run(args.steps)
-    solver.next_dt()  # synchronizes the stream
-    elapsed = time.perf_counter() - t0
-    cells = n ** 3
-    gcells = cells * args.steps / elapsed / 1e9
-    print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
-
-    # per-kernel device times (HIP events on the library's stream)
-    nk = min(args.steps, 20)
-    _, ms_step, ms_red = solver.time_steps(nk)
-    el = 8 if args.precision == "fp64" else 4
-    alg_bytes = 10 * el * cells
-    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
-    finite = bool(np.isfinite(solver.conservation_sums()).all())
-    solver.close()
-
+    r = bench_ensemble(args) if args.workload == "ensemble" else bench_khi(args)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_sample)
-
     out = {
         "metric": "Gcell-updates/s + % HBM peak",
-        "value": round(gcells, 3),
+        "value": round(r["value"], 3),
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "ms_per_step": round(r["s_per_step"] * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "dtype": r["dtype"],
         "data": "synthetic",
-        "config": {"workload": f"khi{n}_{args.precision}", "cells": cells, "periodic": True,
-                   "gamma": 1.4, "cfl": 0.4, "math": args.math, "limiter": "none",
-                   "pct_hbm_peak": round(100.0 * gcells * 1e9 * 10 * el / (HBM_PEAK_GBS * 1e9), 2),
-                   "tile": info, "state_finite": finite},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "step_kernel", "kernel_ms": round(ms_step, 4),
-                     "reduce_kernel_ms": round(ms_red, 4), "alg_bytes_per_launch": alg_bytes},
+        "config": r["config"],
+        "roofline": r["roofline"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -16,28 +16,8 @@
 #include <vector>
 
 #include "../../include/deepgreen.h"
+#include "dg_ctx.h"
 #include "dg_internal.h"
-
-struct dg_ctx {
-    dg_config cfg{};
-    int prec = 1, fast = 0, tile_y = 8, chunk_x = 8;
-    dg::StepArgs args{};
-    size_t n3 = 0, el = 4;
-    void *buf[2][5] = {};
-    int cur = 0;
-    double *partials = nullptr;
-    int nparts_reduce = 0;
-    dg::Scalars *sc = nullptr;      // device
-    dg::Scalars *sc_host = nullptr; // pinned mirror
-    void *stage = nullptr;          // pinned staging (one field)
-    void *scratch = nullptr;        // device scratch (one state) for primitive downloads
-    hipStream_t stream = nullptr;
-    hipGraphExec_t graph = nullptr;
-    int graph_steps = 0;
-    bool have_state = false;
-    bool sums_stale = true;   // the step kernel does not accumulate sums
-    std::string err;
-};
 
 namespace {
 
@@ -180,7 +160,79 @@ int dg_device_count(int *count)
     return DG_OK;
 }
 
-int dg_create(const dg_config *cfg, dg_ctx **out)
+int dg_device_info(int device, char *arch, int len, int *compute_units)
+{
+    if (!arch || len <= 0) return DG_ERR_INVALID;
+    arch[0] = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return DG_ERR_DEVICE;
+    std::string name(prop.gcnArchName);
+    name = name.substr(0, name.find(':'));  // drop feature suffixes (":sramecc+:xnack-")
+    std::snprintf(arch, (size_t)len, "%s", name.c_str());
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    return DG_OK;
+}
+
+int dg_create(const dg_config *cfg, dg_ctx **out) { return dg::create_ctx(cfg, nullptr, out); }
+
+} // extern "C"
+
+namespace dg {
+
+// Workgroup tiling of a context: tile_y rows (4, 8, 16; FP64 4, 8), chunk_x
+// x-planes per workgroup (0: enough chunks for ~2048 workgroups, at least 8
+// planes each).  Drops captured graphs (they bake in the grid).
+int set_tiling(dg_ctx *c, int tile_y, int chunk_x)
+{
+    if (tile_y != 4 && tile_y != 8 && tile_y != 16) return DG_ERR_INVALID;
+    if (c->prec == DG_FP64 && tile_y > 8) return DG_ERR_INVALID;
+    if (chunk_x < 0) return DG_ERR_INVALID;
+    dg::StepArgs a = c->args;
+    a.nby = (a.n[1] + tile_y - 1) / tile_y;
+    a.nbz = (a.n[2] + 63) / 64;
+    int lx = chunk_x;
+    if (lx <= 0) {
+        const long long yz = (long long)a.nby * a.nbz;
+        const long long want = (long long)a.n[0] * yz / 2048;
+        lx = (int)(want < 8 ? 8 : want);
+    }
+    if (lx > a.n[0]) lx = a.n[0];
+    a.lx = lx;
+    a.nbx = (a.n[0] + lx - 1) / lx;
+    const long long nb = (long long)a.nbx * a.nby * a.nbz;
+    if (nb > INT_MAX / 2) return DG_ERR_INVALID;
+    a.nblocks = (int)nb;
+    a.swizzle = (a.nblocks % 8 == 0) ? 1 : 0;
+    c->args = a;
+    c->tile_y = tile_y;
+    c->chunk_x = lx;
+    for (int b = 0; b < 2; ++b)
+        if (c->graph[b]) {
+            hipGraphExecDestroy(c->graph[b]);
+            c->graph[b] = nullptr;
+        }
+    return DG_OK;
+}
+
+// Partial-slot buffer large enough for the step and reduction grids.
+int reserve_partials(dg_ctx *c)
+{
+    const int need = std::max(c->args.nblocks, c->nparts_reduce);
+    if (c->partials && c->partials_cap >= need) return DG_OK;
+    if (c->partials) {
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return DG_ERR_DEVICE;
+        hipFree(c->partials);
+        c->partials = nullptr;
+        c->partials_cap = 0;
+    }
+    if (hipMalloc((void **)&c->partials, (size_t)need * dg::PARTIAL_STRIDE * sizeof(double)) != hipSuccess)
+        return DG_ERR_ALLOC;
+    c->partials_cap = need;
+    return DG_OK;
+}
+
+// Contexts of an ensemble share the ensemble's stream (shared_stream != null).
+int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out)
 {
     if (!cfg || !out) return DG_ERR_INVALID;
     *out = nullptr;
@@ -209,8 +261,6 @@ int dg_create(const dg_config *cfg, dg_ctx **out)
     c->n3 = n3;
     int ty = cfg->tile_y ? cfg->tile_y : 8;
     if (c->prec == DG_FP64 && ty > 8) ty = 8;
-    if (ty != 4 && ty != 8 && ty != 16) { delete c; return DG_ERR_INVALID; }
-    c->tile_y = ty;
 
     dg::StepArgs &a = c->args;
     for (int d = 0; d < 3; ++d) a.n[d] = (int)cfg->n[d];
@@ -220,39 +270,26 @@ int dg_create(const dg_config *cfg, dg_ctx **out)
     a.ds = cfg->ds;
     a.gamma = cfg->gamma;
     a.g = cfg->g;
-    a.nby = (a.n[1] + ty - 1) / ty;
-    a.nbz = (a.n[2] + 63) / 64;
-    int lx = cfg->chunk_x;
-    if (lx <= 0) {
-        const long long yz = (long long)a.nby * a.nbz;
-        long long want = (long long)a.n[0] * yz / 2048;
-        lx = (int)(want < 8 ? 8 : want);
-    }
-    if (lx > a.n[0]) lx = a.n[0];
-    a.lx = lx;
-    c->chunk_x = lx;
-    a.nbx = (a.n[0] + lx - 1) / lx;
-    const long long nb = (long long)a.nbx * a.nby * a.nbz;
-    if (nb > INT_MAX / 2) { delete c; return DG_ERR_INVALID; }
-    a.nblocks = (int)nb;
-    a.swizzle = (a.nblocks % 8 == 0) ? 1 : 0;
     c->nparts_reduce = dg::reduce_blocks(a);
+    if (set_tiling(c, ty, cfg->chunk_x) != DG_OK) { delete c; return DG_ERR_INVALID; }
 
     hipError_t e = hipSetDevice(cfg->device);
     if (e != hipSuccess) { delete c; return DG_ERR_DEVICE; }
-    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) { delete c; return DG_ERR_DEVICE; }
+    if (shared_stream) {
+        c->stream = shared_stream;
+        c->owns_stream = false;
+    } else {
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) { delete c; return DG_ERR_DEVICE; }
+    }
     int rc = DG_OK;
     for (int b = 0; b < 2 && rc == DG_OK; ++b)
         for (int f = 0; f < 5 && rc == DG_OK; ++f)
             if (hipMalloc(&c->buf[b][f], n3 * c->el) != hipSuccess) rc = DG_ERR_ALLOC;
-    const size_t np = (size_t)std::max(a.nblocks, c->nparts_reduce);
-    if (rc == DG_OK && hipMalloc((void **)&c->partials, np * dg::PARTIAL_STRIDE * sizeof(double)) != hipSuccess)
-        rc = DG_ERR_ALLOC;
+    if (rc == DG_OK) rc = reserve_partials(c);
     if (rc == DG_OK && hipMalloc((void **)&c->sc, sizeof(dg::Scalars)) != hipSuccess) rc = DG_ERR_ALLOC;
     if (rc == DG_OK && hipHostMalloc((void **)&c->sc_host, sizeof(dg::Scalars), hipHostMallocDefault) != hipSuccess)
         rc = DG_ERR_ALLOC;
-    if (rc == DG_OK && hipHostMalloc(&c->stage, n3 * c->el, hipHostMallocDefault) != hipSuccess) rc = DG_ERR_ALLOC;
     if (rc != DG_OK) {
         dg_destroy(c);
         return rc;
@@ -268,12 +305,16 @@ int dg_create(const dg_config *cfg, dg_ctx **out)
     *out = c;
     return DG_OK;
 }
+} // namespace dg
+
+extern "C" {
 
 void dg_destroy(dg_ctx *c)
 {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->graph) hipGraphExecDestroy(c->graph);
+    for (int b = 0; b < 2; ++b)
+        if (c->graph[b]) hipGraphExecDestroy(c->graph[b]);
     for (int b = 0; b < 2; ++b)
         for (int f = 0; f < 5; ++f)
             if (c->buf[b][f]) hipFree(c->buf[b][f]);
@@ -282,7 +323,7 @@ void dg_destroy(dg_ctx *c)
     if (c->scratch) hipFree(c->scratch);
     if (c->sc_host) hipHostFree(c->sc_host);
     if (c->stage) hipHostFree(c->stage);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream && c->owns_stream) hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -294,6 +335,7 @@ int dg_set_state(dg_ctx *c, const void *const f[5], int kind)
     HIPCHK(c, hipSetDevice(c->cfg.device));
     const int64_t nx = c->cfg.n[0], ny = c->cfg.n[1], nz = c->cfg.n[2];
     const int64_t gy = ny + 4, gz = nz + 4;
+    if (!c->stage) HIPCHK(c, hipHostMalloc(&c->stage, c->n3 * c->el, hipHostMallocDefault));
     for (int q = 0; q < 5; ++q) {
         HIPCHK(c, hipStreamSynchronize(c->stream));  // staging buffer reuse
         const char *src = static_cast<const char *>(f[q]);
@@ -375,9 +417,11 @@ int dg_reset_clock(dg_ctx *c)
     return DG_OK;
 }
 
+// The captured steps bake in the buffer pointers, so there is one graph per
+// parity of the current buffer.
 static int ensure_graph(dg_ctx *c)
 {
-    if (c->graph) return DG_OK;
+    if (c->graph[c->cur]) return DG_OK;
     hipGraph_t g = nullptr;
     const int cur0 = c->cur;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -387,9 +431,9 @@ static int ensure_graph(dg_ctx *c)
     c->cur = cur0;  // capture launched nothing
     if (rc) { if (g) hipGraphDestroy(g); return rc; }
     if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
-    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&c->graph[cur0], g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
-    if (e != hipSuccess) { c->graph = nullptr; return hip_fail(c, e, "hipGraphInstantiate"); }
+    if (e != hipSuccess) { c->graph[cur0] = nullptr; return hip_fail(c, e, "hipGraphInstantiate"); }
     c->graph_steps = kGraphSteps;
     return DG_OK;
 }
@@ -404,13 +448,13 @@ int dg_run(dg_ctx *c, int64_t max_steps, double t_end, double *t_out, int64_t *s
     const long long start = c->sc_host->steps;
     const long long cap = (max_steps > LLONG_MAX - start) ? LLONG_MAX : start + max_steps;
     hipLaunchKernelGGL(arm_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, t_end, cap, 0);
-    rc = ensure_graph(c);
-    if (rc) return rc;
     const bool timed = std::isfinite(t_end);
     int64_t left = max_steps;
     while (left > 0) {
-        if (left >= c->graph_steps) {
-            HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+        if (left >= kGraphSteps) {
+            rc = ensure_graph(c);
+            if (rc) return rc;
+            HIPCHK(c, hipGraphLaunch(c->graph[c->cur], c->stream));
             left -= c->graph_steps;
         } else {
             rc = launch_one_step(c);

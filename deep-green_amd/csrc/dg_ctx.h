@@ -1,0 +1,40 @@
+// dg_ctx.h -- the context object behind the C ABI (shared by dg_api.cpp and
+// dg_ensemble.cpp).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/deepgreen.h"
+#include "dg_internal.h"
+
+struct dg_ctx {
+    dg_config cfg{};
+    int prec = 1, fast = 0, tile_y = 8, chunk_x = 8;
+    dg::StepArgs args{};
+    size_t n3 = 0, el = 4;
+    void *buf[2][5] = {};
+    int cur = 0;
+    double *partials = nullptr;
+    int partials_cap = 0;     // slots allocated
+    int nparts_reduce = 0;
+    dg::Scalars *sc = nullptr;      // device
+    dg::Scalars *sc_host = nullptr; // pinned mirror
+    void *stage = nullptr;          // pinned staging (one field), allocated at first upload
+    void *scratch = nullptr;        // device scratch (one state) for primitive downloads
+    hipStream_t stream = nullptr;
+    bool owns_stream = true;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // dg_run replay, by state buffer parity
+    int graph_steps = 0;
+    bool have_state = false;
+    bool sums_stale = true;   // the step kernel does not accumulate sums
+    std::string err;
+};
+
+
+namespace dg {
+int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out);
+int set_tiling(dg_ctx *c, int tile_y, int chunk_x);
+int reserve_partials(dg_ctx *c);
+}

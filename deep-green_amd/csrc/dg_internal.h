@@ -42,6 +42,21 @@ struct Scalars {
 
 template <typename T> struct Fields { T *f[5]; };
 
+// One grid of a batched ensemble (device-resident table).
+struct EnsInstance {
+    StepArgs args;       // grid, boundaries, tiling (lx, nby, nbz) of this instance
+    void *buf[2][5];     // state ping-pong buffers
+    Scalars *sc;         // this instance's scalars
+    int part_base;       // first partial slot
+    int nparts;          // partial slots (= tiles)
+    double cfl;
+    int nan_fallback;
+    int pad_;
+};
+
+// One (instance, tile) work item of the persistent ensemble kernel.
+struct EnsItem { int inst, bx, by, bz, slot; };
+
 // Kernel launch entry points (dg_step.hip).  `prec` is 0 FP64, 1 FP32,
 // 2 FP32_BULK; `fast` selects the reassociated arithmetic.
 int launch_step(int prec, int fast, int tile_y, const StepArgs &a, const void *const in[5],
@@ -54,6 +69,12 @@ int launch_finalize(int prec, const StepArgs &a, Scalars *sc, const double *part
 int launch_prim_to_cons(int prec, const StepArgs &a, void *const u[5], hipStream_t s);
 int launch_cons_to_prim(int prec, const StepArgs &a, const void *const u[5], void *const w[5],
                         hipStream_t s);
+int launch_ens_step(int prec, int fast, int tile_y, int limiter, const EnsInstance *inst, const EnsItem *items,
+                    int nitems, int grid, int parity, double *partials, hipStream_t s);
+int launch_ens_finalize(int prec, const EnsInstance *inst, int ninst, const double *partials, int mode,
+                        hipStream_t s);
+// Resident workgroups of the ensemble step kernel per CU (-1 on error).
+int ens_blocks_per_cu(int prec, int fast, int tile_y, int limiter);
 int step_lds_bytes(int prec, int tile_y);
 int reduce_blocks(const StepArgs &a);
 

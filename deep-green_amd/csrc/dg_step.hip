@@ -521,33 +521,20 @@ __device__ __forceinline__ void block_partials(double den, int nanf, const doubl
     }
 }
 
+// One tile (TY x 64 columns, planes [bx*lx, min(bx*lx+lx, nx))) of one
+// grid: the whole fused step for those cells; writes the tile's partial
+// (max wave-speed denominator, NaN flag) to partial_out.  Called by the
+// single-grid kernel and by the persistent ensemble kernel.
 template <typename T, int TY, bool FAST, bool BULK, int LIM>
-__global__ void __launch_bounds__(TY * TZ)
-step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, const Scalars *__restrict__ sc,
-            double *__restrict__ partials)
+__device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const StepArgs &A,
+                                          const Fields<const T> &U, const Fields<T> &Vout,
+                                          const Scalars *__restrict__ sc, double *__restrict__ partial_out,
+                                          int bx, int by, int bz)
 {
     constexpr int NT = TY * TZ;
-    __shared__ StepLDS<T, TY> L;
-    __shared__ double red[(NT / 64) * 8];
-
     const int tid = threadIdx.x;
     const int ty = tid >> 6, tz = tid & 63;
     const int nx = A.n[0], ny = A.n[1], nz = A.n[2];
-#ifdef DG_EMU_POISON
-    {   // CPU-emulation builds only: make every unwritten LDS read visible
-        T *raw = reinterpret_cast<T *>(&L);
-        for (size_t q = tid; q < sizeof(L) / sizeof(T); q += NT) raw[q] = T(NAN);
-        __syncthreads();
-    }
-#endif
-
-    // tile of this workgroup (optionally spread so each XCD gets a
-    // contiguous range of neighbouring tiles)
-    int b = blockIdx.x;
-    if (A.swizzle) b = (b & 7) * (A.nblocks >> 3) + (b >> 3);
-    const int bz = b % A.nbz;
-    const int by = (b / A.nbz) % A.nby;
-    const int bx = b / (A.nbz * A.nby);
     const int x0 = bx * A.lx, y0 = by * TY, z0 = bz * TZ;
     const int xend = min(x0 + A.lx, nx);
     const int yc = y0 + ty, zc = z0 + tz;
@@ -777,7 +764,62 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
 
     if constexpr (!BULK) dmaxden = (double)maxden;
     const double nosums[5] = {0, 0, 0, 0, 0};
-    block_partials<NT>(dmaxden, nanf, nosums, red, partials, blockIdx.x, tid);
+    block_partials<NT>(dmaxden, nanf, nosums, red, partial_out, 0, tid);
+}
+
+// Occupancy target: the FP32 fast kernels fit 128 VGPRs (4 waves per SIMD).
+template <typename T, bool FAST> constexpr int waves_per_eu() { return (FAST && sizeof(T) == 4) ? 4 : 1; }
+
+template <typename T, int TY, bool FAST, bool BULK, int LIM>
+__global__ void __launch_bounds__(TY * TZ) __attribute__((amdgpu_waves_per_eu(waves_per_eu<T, FAST>())))
+step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, const Scalars *__restrict__ sc,
+            double *__restrict__ partials)
+{
+    __shared__ StepLDS<T, TY> L;
+    __shared__ double red[(TY * TZ / 64) * 8];
+#ifdef DG_EMU_POISON
+    {   // CPU-emulation builds only: make every unwritten LDS read visible
+        T *raw = reinterpret_cast<T *>(&L);
+        for (size_t q = threadIdx.x; q < sizeof(L) / sizeof(T); q += TY * TZ) raw[q] = T(NAN);
+        __syncthreads();
+    }
+#endif
+    // tile of this workgroup (optionally spread so each XCD gets a
+    // contiguous range of neighbouring tiles)
+    int b = blockIdx.x;
+    if (A.swizzle) b = (b & 7) * (A.nblocks >> 3) + (b >> 3);
+    const int bz = b % A.nbz;
+    const int by = (b / A.nbz) % A.nby;
+    const int bx = b / (A.nbz * A.nby);
+    step_tile<T, TY, FAST, BULK, LIM>(L, red, A, U, Vout, sc, partials + (size_t)blockIdx.x * PARTIAL_STRIDE,
+                                      bx, by, bz);
+}
+
+// Persistent ensemble step: workgroups stride over a work list of
+// (instance, tile) items (sorted by cost, largest first); every instance has
+// its own grid, state buffers, scalars and partial slots.
+template <typename T, int TY, bool FAST, bool BULK, int LIM>
+__global__ void __launch_bounds__(TY * TZ) __attribute__((amdgpu_waves_per_eu(waves_per_eu<T, FAST>())))
+ens_step_kernel(const EnsInstance *__restrict__ inst, const EnsItem *__restrict__ items, int nitems, int parity,
+                double *__restrict__ partials)
+{
+    __shared__ StepLDS<T, TY> L;
+    __shared__ double red[(TY * TZ / 64) * 8];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const EnsItem w = items[it];
+        const EnsInstance &I = inst[w.inst];
+        const StepArgs A = I.args;
+        Fields<const T> U;
+        Fields<T> V;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            U.f[f] = static_cast<const T *>(I.buf[parity][f]);
+            V.f[f] = static_cast<T *>(I.buf[1 - parity][f]);
+        }
+        step_tile<T, TY, FAST, BULK, LIM>(L, red, A, U, V, I.sc, partials + (size_t)w.slot * PARTIAL_STRIDE,
+                                          w.bx, w.by, w.bz);
+        __syncthreads();   // LDS and the reduction scratch are reused by the next item
+    }
 }
 
 // --------------------------------------------------------- small kernels --
@@ -813,18 +855,16 @@ reduce_kernel(const StepArgs A, const Fields<const T> U, double *__restrict__ pa
 
 // dt from the partials (calc_timestep, physics.rs:177-196: CFL * min over
 // cells of ds / den == CFL * ds / max den), conservation sums, and the run
-// clock (main.rs:296-298).  One workgroup; fixed reduction order.
-// mode 0: dt and sums of a fresh state; 1: after a step (advance the clock,
-// new dt); 2: sums only.
-template <typename T, bool BULK>
-__global__ void __launch_bounds__(1024)
-finalize_kernel(const StepArgs A, Scalars *__restrict__ sc, const double *__restrict__ partials, int np,
-                double cfl, int nan_fallback, int mode)
+// clock (main.rs:296-298).  One workgroup of NT threads; fixed reduction
+// order.  mode 0: dt and sums of a fresh state; 1: after a step (advance the
+// clock, new dt); 2: sums only.
+template <typename T, bool BULK, int NT>
+__device__ __forceinline__ void finalize_body(double ds, Scalars *__restrict__ sc, const double *__restrict__ partials,
+                                              int np, double cfl, int nan_fallback, int mode, double *red)
 {
-    __shared__ double red[16 * 8];
     const int tid = threadIdx.x;
     double den = 0.0, nanf = 0.0, s[5] = {0, 0, 0, 0, 0};
-    for (int i = tid; i < np; i += 1024) {
+    for (int i = tid; i < np; i += NT) {
         const double *P = partials + (size_t)i * PARTIAL_STRIDE;
         den = fmax(den, P[0]);
         nanf = fmax(nanf, P[1]);
@@ -842,7 +882,7 @@ finalize_kernel(const StepArgs A, Scalars *__restrict__ sc, const double *__rest
     if (tid != 0) return;
     double o[7];
     for (int q = 0; q < 7; ++q) o[q] = red[q];
-    for (int w = 1; w < 16; ++w) {
+    for (int w = 1; w < NT / 64; ++w) {
         o[0] = fmax(o[0], red[w * 8 + 0]);
         o[1] = fmax(o[1], red[w * 8 + 1]);
         for (int q = 2; q < 7; ++q) o[q] += red[w * 8 + q];
@@ -860,10 +900,10 @@ finalize_kernel(const StepArgs A, Scalars *__restrict__ sc, const double *__rest
     const bool nan = o[1] != 0.0;
     double dt;
     if constexpr (!BULK) {
-        const T minval = nan ? (nan_fallback ? T(1.0) : T(NAN)) : T(A.ds) / T(o[0]);
+        const T minval = nan ? (nan_fallback ? T(1.0) : T(NAN)) : T(ds) / T(o[0]);
         dt = (double)(T(cfl) * minval);
     } else {
-        const double minval = nan ? (nan_fallback ? 1.0 : NAN) : A.ds / o[0];
+        const double minval = nan ? (nan_fallback ? 1.0 : NAN) : ds / o[0];
         dt = cfl * minval;
     }
     sc->dt = dt;
@@ -872,6 +912,26 @@ finalize_kernel(const StepArgs A, Scalars *__restrict__ sc, const double *__rest
     if (mode == 0)
         for (int q = 0; q < 5; ++q) sc->sums[q] = o[2 + q];
     sc->active = !((sc->t > sc->t_end) || (sc->steps >= sc->max_steps));
+}
+
+template <typename T, bool BULK>
+__global__ void __launch_bounds__(1024)
+finalize_kernel(const StepArgs A, Scalars *__restrict__ sc, const double *__restrict__ partials, int np,
+                double cfl, int nan_fallback, int mode)
+{
+    __shared__ double red[16 * 8];
+    finalize_body<T, BULK, 1024>(A.ds, sc, partials, np, cfl, nan_fallback, mode, red);
+}
+
+// One workgroup per ensemble instance: its partial slots -> its scalars.
+template <typename T, bool BULK>
+__global__ void __launch_bounds__(256)
+ens_finalize_kernel(const EnsInstance *__restrict__ inst, const double *__restrict__ partials, int mode)
+{
+    __shared__ double red[4 * 8];
+    const EnsInstance &I = inst[blockIdx.x];
+    finalize_body<T, BULK, 256>(I.args.ds, I.sc, partials + (size_t)I.part_base * PARTIAL_STRIDE, I.nparts, I.cfl,
+                                I.nan_fallback, mode, red);
 }
 
 // calc_conserved (physics.rs:143-175), in place on interior arrays holding W.
@@ -958,6 +1018,88 @@ int launch_step(int prec, int fast, int tile_y, const StepArgs &a, const void *c
         } else return -1;
     }
 #undef DG_CASE
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <typename T, int TY, bool FAST, bool BULK>
+static void launch_ens_t(const EnsInstance *inst, const EnsItem *items, int nitems, int grid, int parity,
+                         int limiter, double *partials, hipStream_t s)
+{
+    const dim3 g(grid), b(TY * TZ);
+    if (limiter == LIM_MINMOD)
+        hipLaunchKernelGGL((ens_step_kernel<T, TY, FAST, BULK, LIM_MINMOD>), g, b, 0, s, inst, items, nitems, parity, partials);
+    else if (limiter == LIM_VANLEER)
+        hipLaunchKernelGGL((ens_step_kernel<T, TY, FAST, BULK, LIM_VANLEER>), g, b, 0, s, inst, items, nitems, parity, partials);
+    else
+        hipLaunchKernelGGL((ens_step_kernel<T, TY, FAST, BULK, LIM_NONE>), g, b, 0, s, inst, items, nitems, parity, partials);
+}
+
+int launch_ens_step(int prec, int fast, int tile_y, int limiter, const EnsInstance *inst, const EnsItem *items,
+                    int nitems, int grid, int parity, double *partials, hipStream_t s)
+{
+#define DG_ENS(T, TYV, F, B) launch_ens_t<T, TYV, F, B>(inst, items, nitems, grid, parity, limiter, partials, s)
+    if (prec == 0) {
+        if (tile_y == 4) DG_ENS(double, 4, false, false);
+        else if (tile_y == 8) DG_ENS(double, 8, false, false);
+        else return -1;
+    } else {
+        const bool bulk = prec == 2;
+        if (tile_y == 4) {
+            if (fast) { if (bulk) DG_ENS(float, 4, true, true); else DG_ENS(float, 4, true, false); }
+            else { if (bulk) DG_ENS(float, 4, false, true); else DG_ENS(float, 4, false, false); }
+        } else if (tile_y == 8) {
+            if (fast) { if (bulk) DG_ENS(float, 8, true, true); else DG_ENS(float, 8, true, false); }
+            else { if (bulk) DG_ENS(float, 8, false, true); else DG_ENS(float, 8, false, false); }
+        } else if (tile_y == 16) {
+            if (fast) { if (bulk) DG_ENS(float, 16, true, true); else DG_ENS(float, 16, true, false); }
+            else { if (bulk) DG_ENS(float, 16, false, true); else DG_ENS(float, 16, false, false); }
+        } else return -1;
+    }
+#undef DG_ENS
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <typename T, int TY, bool FAST, bool BULK>
+static int ens_occ_t(int limiter)
+{
+    int n = 0;
+    hipError_t e;
+    if (limiter == LIM_MINMOD)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ens_step_kernel<T, TY, FAST, BULK, LIM_MINMOD>, TY * TZ, 0);
+    else if (limiter == LIM_VANLEER)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ens_step_kernel<T, TY, FAST, BULK, LIM_VANLEER>, TY * TZ, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ens_step_kernel<T, TY, FAST, BULK, LIM_NONE>, TY * TZ, 0);
+    return e == hipSuccess ? n : -1;
+}
+
+int ens_blocks_per_cu(int prec, int fast, int tile_y, int limiter)
+{
+    if (prec == 0) {
+        if (tile_y == 4) return ens_occ_t<double, 4, false, false>(limiter);
+        if (tile_y == 8) return ens_occ_t<double, 8, false, false>(limiter);
+        return -1;
+    }
+    const bool bulk = prec == 2;
+#define DG_OCC(TYV)                                                                                    \
+    if (tile_y == TYV) {                                                                              \
+        if (fast) return bulk ? ens_occ_t<float, TYV, true, true>(limiter) : ens_occ_t<float, TYV, true, false>(limiter); \
+        return bulk ? ens_occ_t<float, TYV, false, true>(limiter) : ens_occ_t<float, TYV, false, false>(limiter); \
+    }
+    DG_OCC(4) DG_OCC(8) DG_OCC(16)
+#undef DG_OCC
+    return -1;
+}
+
+int launch_ens_finalize(int prec, const EnsInstance *inst, int ninst, const double *partials, int mode,
+                        hipStream_t s)
+{
+    if (prec == 0)
+        hipLaunchKernelGGL((ens_finalize_kernel<double, false>), dim3(ninst), dim3(256), 0, s, inst, partials, mode);
+    else if (prec == 1)
+        hipLaunchKernelGGL((ens_finalize_kernel<float, false>), dim3(ninst), dim3(256), 0, s, inst, partials, mode);
+    else
+        hipLaunchKernelGGL((ens_finalize_kernel<float, true>), dim3(ninst), dim3(256), 0, s, inst, partials, mode);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -33,7 +33,11 @@ STATUS = {0: "ok", -1: "invalid argument", -2: "device error", -3: "allocation f
 # every entry point declared in include/deepgreen.h
 EXPORTS = ("dg_config_default", "dg_create", "dg_destroy", "dg_last_error", "dg_set_state",
            "dg_get_state", "dg_step", "dg_run", "dg_reset_clock", "dg_conservation_sums",
-           "dg_next_dt", "dg_device_state", "dg_time_steps", "dg_kernel_info", "dg_device_count")
+           "dg_next_dt", "dg_device_state", "dg_time_steps", "dg_kernel_info", "dg_device_count",
+           "dg_device_info",
+           "dg_ensemble_create", "dg_ensemble_destroy", "dg_ensemble_last_error", "dg_ensemble_set_state",
+           "dg_ensemble_get_state", "dg_ensemble_run", "dg_ensemble_clock", "dg_ensemble_conservation_sums",
+           "dg_ensemble_time_steps", "dg_ensemble_retile", "dg_ensemble_info")
 
 
 class DgConfig(ctypes.Structure):
@@ -89,8 +93,23 @@ def load_library(path=None):
     L.dg_time_steps.argtypes = [vp, i64, P(d), P(d), P(d)]
     L.dg_kernel_info.argtypes = [vp, P(i32), P(i32), P(i64), P(i32), P(i32)]
     L.dg_device_count.argtypes = [P(i32)]
+    L.dg_ensemble_create.argtypes = [P(DgConfig), i32, i32, P(vp)]
+    L.dg_ensemble_destroy.argtypes = [vp]
+    L.dg_ensemble_destroy.restype = None
+    L.dg_ensemble_last_error.argtypes = [vp]
+    L.dg_ensemble_last_error.restype = ctypes.c_char_p
+    L.dg_ensemble_set_state.argtypes = [vp, i32, P(vp), i32]
+    L.dg_ensemble_get_state.argtypes = [vp, i32, P(vp), i32]
+    L.dg_ensemble_run.argtypes = [vp, i64, d]
+    L.dg_ensemble_clock.argtypes = [vp, i32, P(d), P(i64), P(d)]
+    L.dg_ensemble_conservation_sums.argtypes = [vp, i32, P(d)]
+    L.dg_ensemble_time_steps.argtypes = [vp, i64, P(d), P(d)]
+    L.dg_ensemble_retile.argtypes = [vp, i32, P(ctypes.c_int32), i32]
+    L.dg_ensemble_info.argtypes = [vp, P(i64), P(i32), P(i32), P(i32)]
+    L.dg_device_info.argtypes = [i32, ctypes.c_char_p, i32, P(i32)]
     for name in EXPORTS:
-        if name not in ("dg_config_default", "dg_destroy", "dg_last_error"):
+        if name not in ("dg_config_default", "dg_destroy", "dg_last_error", "dg_ensemble_destroy",
+                        "dg_ensemble_last_error"):
             getattr(L, name).restype = ctypes.c_int
     if path is None:
         _lib = L
@@ -263,7 +282,118 @@ class Solver:
         return a[2:-2, 2:-2, 2:-2]
 
 
+def _field_ptrs(sim, arrays, write=False):
+    dt = sim.dtype
+    out = []
+    for a in arrays:
+        if a.shape != sim.shape or a.dtype != dt or not a.flags.c_contiguous:
+            raise ValueError(f"expected C-contiguous {dt.__name__} arrays of shape {sim.shape}")
+        if write and not a.flags.writeable:
+            raise ValueError("output arrays must be writeable")
+        out.append(a.ctypes.data)
+    return (ctypes.c_void_p * 5)(*out)
+
+
+class Ensemble:
+    """Many independent grids advanced together by one persistent step
+    launch per step (dg_ensemble_*).  `sims` share precision, math, limiter,
+    device and tile_y; everything else is per instance.  Each instance
+    follows exactly the trajectory a Solver of the same Simulation follows."""
+
+    def __init__(self, sims, grid=0, lib=None):
+        self.sims = list(sims)
+        self._L = lib or load_library()
+        self._h = ctypes.c_void_p()
+        cfgs = (DgConfig * len(self.sims))()
+        for i, s in enumerate(self.sims):
+            cfgs[i] = s.config(self._L)
+        rc = self._L.dg_ensemble_create(cfgs, len(self.sims), int(grid), ctypes.byref(self._h))
+        if rc != 0:
+            raise DeepGreenError(f"dg_ensemble_create failed: {STATUS.get(rc, rc)}")
+
+    def __len__(self):
+        return len(self.sims)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.dg_ensemble_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._L.dg_ensemble_last_error(self._h)
+            raise DeepGreenError(f"{what}: {STATUS.get(rc, rc)} ({msg.decode() if msg else ''})")
+
+    def set_state(self, i, fields, kind=PRIMITIVE):
+        sim = self.sims[i]
+        arrs = [np.ascontiguousarray(f, dtype=sim.dtype) for f in fields]
+        self._check(self._L.dg_ensemble_set_state(self._h, int(i), _field_ptrs(sim, arrs), int(kind)),
+                    f"dg_ensemble_set_state({i})")
+
+    def get_state(self, i, kind=CONSERVED):
+        sim = self.sims[i]
+        arrs = [np.empty(sim.shape, dtype=sim.dtype) for _ in range(5)]
+        self._check(self._L.dg_ensemble_get_state(self._h, int(i), _field_ptrs(sim, arrs, True), int(kind)),
+                    f"dg_ensemble_get_state({i})")
+        return arrs
+
+    def run(self, max_steps, t_end=math.inf):
+        self._check(self._L.dg_ensemble_run(self._h, int(max_steps), float(t_end)), "dg_ensemble_run")
+
+    def clock(self, i):
+        """(t, steps, next dt) of instance i."""
+        t, n, dt = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+        self._check(self._L.dg_ensemble_clock(self._h, int(i), ctypes.byref(t), ctypes.byref(n), ctypes.byref(dt)),
+                    "dg_ensemble_clock")
+        return t.value, n.value, dt.value
+
+    def conservation_sums(self, i):
+        out = (ctypes.c_double * 5)()
+        self._check(self._L.dg_ensemble_conservation_sums(self._h, int(i), out), "dg_ensemble_conservation_sums")
+        return np.array(out[:])
+
+    def time_steps(self, nsteps):
+        """(mean ms of the persistent step kernel, mean ms of the dt kernel)
+        over nsteps steps, from HIP events on the ensemble's stream."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self._L.dg_ensemble_time_steps(self._h, int(nsteps), ctypes.byref(a), ctypes.byref(b)),
+                    "dg_ensemble_time_steps")
+        return a.value, b.value
+
+    def set_grid(self, grid):
+        """Persistent workgroups of the step launch (0: device maximum)."""
+        self._check(self._L.dg_ensemble_set_grid(self._h, int(grid)), "dg_ensemble_retile")
+
+    def info(self):
+        cells, items, grid, ty = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.dg_ensemble_info(self._h, ctypes.byref(cells), ctypes.byref(items), ctypes.byref(grid),
+                                             ctypes.byref(ty)), "dg_ensemble_info")
+        return dict(cells=cells.value, items=items.value, grid=grid.value, tile_y=ty.value)
+
+
 def device_count():
     n = ctypes.c_int(0)
     load_library().dg_device_count(ctypes.byref(n))
     return n.value
+
+
+def device_info(device=0, lib=None):
+    """(architecture name such as "gfx950", compute units) of a HIP device."""
+    buf = ctypes.create_string_buffer(64)
+    cus = ctypes.c_int(0)
+    rc = (lib or load_library()).dg_device_info(int(device), buf, 64, ctypes.byref(cus))
+    if rc != 0:
+        raise DeepGreenError(f"dg_device_info: {STATUS.get(rc, rc)}")
+    return buf.value.decode(), cus.value

@@ -137,6 +137,56 @@ int dg_kernel_info(dg_ctx *ctx, int *tile_y, int *chunk_x, int64_t *workgroups,
 
 int dg_device_count(int *count);
 
+/* Architecture name of a HIP device (e.g. "gfx950", NUL-terminated in arch)
+ * and its number of compute units. */
+int dg_device_info(int device, char *arch, int len, int *compute_units);
+
+/* ---------------------------------------------------------------- ensemble --
+ * Many independent grids advanced together: one persistent step kernel per
+ * step strides over the (instance, tile) work items of every instance,
+ * followed by one dt/clock kernel with a workgroup per instance; dg_ensemble_run
+ * replays these from a captured HIP graph.  No reference counterpart: the
+ * reference steps one grid per process (main.rs:283-299).  Every instance
+ * computes exactly what dg_step / dg_run compute for the same dg_config and
+ * state (same kernels, bit for bit), whatever the tiling. */
+typedef struct dg_ensemble dg_ensemble;
+
+/* cfgs[0..count-1]: one configuration per instance.  precision, math,
+ * limiter, device and tile_y must agree across instances (tile_y 0 = 8);
+ * n, bc, ds, gamma, cfl, g, gravity, dt_nan_fallback and chunk_x are per
+ * instance.  grid: persistent workgroups per step launch (0 = as many as fit
+ * on the device at once). */
+int dg_ensemble_create(const dg_config *cfgs, int count, int grid, dg_ensemble **out);
+void dg_ensemble_destroy(dg_ensemble *ens);
+const char *dg_ensemble_last_error(const dg_ensemble *ens);
+
+/* dg_set_state / dg_get_state of instance i. */
+int dg_ensemble_set_state(dg_ensemble *ens, int i, const void *const f[5], int kind);
+int dg_ensemble_get_state(dg_ensemble *ens, int i, void *const f[5], int kind);
+
+/* Up to max_steps steps of every instance; an instance whose own clock
+ * passes t_end (the test of dg_run) stops advancing while the others go on. */
+int dg_ensemble_run(dg_ensemble *ens, int64_t max_steps, double t_end);
+
+/* Run clock of instance i (as dg_run reports it) and the dt of its next step. */
+int dg_ensemble_clock(dg_ensemble *ens, int i, double *t, int64_t *steps, double *next_dt);
+
+/* dg_conservation_sums of instance i. */
+int dg_ensemble_conservation_sums(dg_ensemble *ens, int i, double out[5]);
+
+/* Timing harness: nsteps steps launched one by one with HIP events around
+ * every step kernel; mean device time per step kernel and per dt kernel. */
+int dg_ensemble_time_steps(dg_ensemble *ens, int64_t nsteps, double *ms_step_kernel, double *ms_dt_kernel);
+
+/* New tiling without touching the state: tile_y for all instances (0 = keep),
+ * chunk_x[i] per instance (chunk_x NULL = keep; 0 = automatic), grid
+ * persistent workgroups (0 = as many as fit).  Results do not change. */
+int dg_ensemble_retile(dg_ensemble *ens, int tile_y, const int32_t *chunk_x, int grid);
+
+/* Interior cells over all instances, work items per step, persistent
+ * workgroups, rows per workgroup. */
+int dg_ensemble_info(dg_ensemble *ens, int64_t *cells, int *items, int *grid, int *tile_y);
+
 #ifdef __cplusplus
 }
 #endif

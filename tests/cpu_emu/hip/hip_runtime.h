@@ -2,16 +2,17 @@
 // TEST INFRASTRUCTURE ONLY: lets g++ compile the HIP kernels for the host so
 // their logic (indexing, barriers, shuffles, boundary rules) can be checked
 // bit for bit against the oracle on machines without a GPU.  Each workgroup
-// runs as blockDim host threads with a real barrier; blocks run in sequence.
+// runs as blockDim cooperative fibers on one host thread (a barrier yields to
+// the next fiber; a full round of yields is a full barrier); blocks run in
+// sequence.
 #pragma once
-#include <pthread.h>
 
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
-#include <thread>
+#include <tuple>
 #include <vector>
 
 #define __global__
@@ -19,6 +20,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define amdgpu_waves_per_eu(...) unused
 #define __shared__ static
 
 struct dim3 {
@@ -26,13 +28,12 @@ struct dim3 {
     dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
 };
 struct emu_idx { unsigned x = 0, y = 0, z = 0; };
-extern thread_local emu_idx threadIdx, blockIdx;
+extern emu_idx threadIdx, blockIdx;
 extern emu_idx blockDim, gridDim;
 
 namespace emu {
-extern pthread_barrier_t bar;
 extern double xbuf[1024];
-inline void sync() { pthread_barrier_wait(&bar); }
+void sync();
 template <typename T> T exch(T v, int src_lane_delta, int mode)
 {
     const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid & ~63;
@@ -83,7 +84,8 @@ typedef void *hipGraph_t;
 typedef void *hipGraphExec_t;
 enum { hipSuccess = 0, hipErrorUnknown = 999 };
 enum { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipStreamNonBlocking = 1,
-       hipHostMallocDefault = 0, hipStreamCaptureModeThreadLocal = 0 };
+       hipHostMallocDefault = 0, hipStreamCaptureModeThreadLocal = 0,
+       hipDeviceAttributeMultiprocessorCount = 16 };
 typedef int hipMemcpyKind;
 
 namespace emu {
@@ -92,9 +94,12 @@ extern bool capturing;
 extern std::vector<std::function<void()>> captured;
 }  // namespace emu
 
+// Arguments are evaluated at launch (as HIP does), so a captured launch
+// replays with the values it was recorded with.
 #define hipLaunchKernelGGL(K, G, B, SH, S, ...)                                          \
     do {                                                                                 \
-        auto emu_body_ = [=]() { K(__VA_ARGS__); };                                      \
+        auto emu_args_ = std::make_tuple(__VA_ARGS__);                                   \
+        auto emu_body_ = [=]() { std::apply(K, emu_args_); };                            \
         const unsigned g_ = dim3(G).x, b_ = dim3(B).x;                                   \
         if (emu::capturing) emu::captured.push_back([=]() { emu::launch(g_, b_, emu_body_); }); \
         else emu::launch(g_, b_, emu_body_);                                             \
@@ -141,6 +146,21 @@ inline hipError_t hipGraphExecDestroy(hipGraphExec_t g) { delete static_cast<std
 inline hipError_t hipGraphLaunch(hipGraphExec_t g, hipStream_t)
 {
     for (auto &f : *static_cast<std::vector<std::function<void()>> *>(g)) f();
+    return hipSuccess;
+}
+// A small device: 4 CUs, one resident workgroup per CU (persistent kernels
+// then stride over many items per workgroup).
+inline hipError_t hipDeviceGetAttribute(int *v, int, int) { *v = 4; return hipSuccess; }
+struct hipDeviceProp_t { char gcnArchName[256]; int multiProcessorCount; };
+inline hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int)
+{
+    std::strcpy(p->gcnArchName, "emulated");
+    p->multiProcessorCount = 4;
+    return hipSuccess;
+}
+template <typename F> inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, F, int, size_t)
+{
+    *n = 1;
     return hipSuccess;
 }
 inline int min(int a, int b) { return a < b ? a : b; }

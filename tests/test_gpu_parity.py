@@ -131,6 +131,24 @@ def test_graph_run_equals_single_steps():
     np.testing.assert_array_equal(s1, s2)
 
 
+def test_graph_runs_across_buffer_parity():
+    """run(17) leaves the state in the other buffer (one single launch after a
+    graph replay); the next run(16) must replay the graph captured for that
+    parity.  Equals 33 single steps."""
+    from deepgreen import Simulation, Solver
+    c = cases.build_case("smooth_mixed")
+    c["nsteps"] = 33
+    U1, _, _ = lib_run(c, "fp32", per_step=True)
+    prm = {k: v for k, v in cases.PARAMS[c["params"]].items() if k != "gravity_k_shift"}
+    with Solver(Simulation(*c["interior"], c["ds"], c["bcs"], precision="fp32", **prm)) as s:
+        s.set_primitives(c["w"])
+        s.run(17)
+        t, n = s.run(16)
+        assert n == 33
+        U2 = np.stack([np.array(s.interior(a)) for a in s.get_state()])
+    np.testing.assert_array_equal(U1, U2)
+
+
 def test_conservation_sums_and_determinism():
     c = cases.build_case("blast16")
     U1, _, s1 = lib_run(c, "fp64", per_step=False)

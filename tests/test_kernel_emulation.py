@@ -1,5 +1,5 @@
 """The HIP step kernels compiled for the CPU (tests/cpu_emu: each workgroup
-runs as host threads with real barriers; LDS is poisoned with NaN at kernel
+runs as cooperative host fibers with real barriers; LDS is poisoned with NaN at kernel
 start so a read of an unwritten slot shows) and checked bit for bit against
 the oracle on small grids.  Covers the kernel logic -- tiling, halos,
 boundary maps, slot rotation, chunk seams -- without a GPU; the same source
@@ -87,3 +87,62 @@ def test_emulated_periodic_gravity(emu_lib):
     U, dts, Uo, dto = run_both(emu_lib, interior, w, ds, [cases.PERIODIC] * 6, 2, "fp64", params="native")
     np.testing.assert_array_equal(dts, dto.astype(np.float64))
     np.testing.assert_array_equal(U, Uo)
+
+
+def oracle_run(interior, w, ds, bcs, nsteps, limiter="none"):
+    o = Oracle(interior, ds, bcs, precision="f32", limiter=limiter, **cases.PARAMS["cfg"])
+    o.set_primitives(w)
+    n, t, _ = o.compute(1e30, nsteps)
+    U = np.stack([np.array(o.interior_view(a)) for a in o.cons])
+    o.calc_primitives()
+    o.update_primitive_ghost_cells()
+    return U, t, n, o.calc_timestep()
+
+
+def test_emulated_ensemble_matches_oracle(emu_lib):
+    """Three uneven grids in one ensemble (persistent kernel striding over the
+    work list of all instances, four workgroups in the emulation), run for
+    17 + 16 steps: a captured graph, a single step that flips the buffer
+    parity, then the graph of the other parity.  Each instance is bitwise the
+    oracle's trajectory (state, clock, next dt), which the standalone
+    context also reproduces bit for bit."""
+    import deepgreen
+    prm = {k: v for k, v in cases.PARAMS["cfg"].items() if k != "gravity_k_shift"}
+    members = [((6, 5, 7), [cases.OUTFLOW] * 6, 2),
+               ((5, 7, 4), [cases.PERIODIC, cases.PERIODIC, cases.OUTFLOW, cases.OUTFLOW, cases.WALL, cases.WALL], 0),
+               ((4, 4, 9), [cases.PERIODIC] * 6, 3)]
+    sims, states = [], []
+    for k, (interior, bcs, chunk) in enumerate(members):
+        w, ds = cases.smooth_random(interior, seed=10 + k)
+        sims.append(deepgreen.Simulation(*interior, ds, bcs, precision="fp32", math="exact", limiter="minmod",
+                                         tile_y=4, chunk_x=chunk, **prm))
+        states.append(w)
+    with deepgreen.Ensemble(sims, lib=emu_lib) as ens:
+        assert ens.info()["grid"] == 4
+        for i, w in enumerate(states):
+            ens.set_state(i, w)
+        ens.run(17)
+        ens.run(16)
+        for i, (interior, bcs, _) in enumerate(members):
+            got = np.stack([a[2:-2, 2:-2, 2:-2] for a in ens.get_state(i)])
+            U, t, n, dt_next = oracle_run(interior, states[i], sims[i].ds, bcs, 33, limiter="minmod")
+            np.testing.assert_array_equal(got, U)
+            assert ens.clock(i) == (t, 33, dt_next) and n == 33
+
+
+def test_emulated_run_graph_parity(emu_lib):
+    """dg_run: 17 steps (one captured graph and one single step) then 16 (the
+    graph captured for the other buffer parity) equal 33 oracle steps."""
+    import deepgreen
+    prm = {k: v for k, v in cases.PARAMS["cfg"].items() if k != "gravity_k_shift"}
+    interior, bcs = (4, 5, 3), [cases.OUTFLOW] * 6
+    w, ds = cases.smooth_random(interior, seed=21)
+    sim = deepgreen.Simulation(*interior, ds, bcs, precision="fp32", tile_y=4, **prm)
+    with deepgreen.Solver(sim, lib=emu_lib) as s:
+        s.set_primitives(w)
+        s.run(17)
+        t, n = s.run(16)
+        got = np.stack([a[2:-2, 2:-2, 2:-2] for a in s.get_state()])
+    U, to, no, _ = oracle_run(interior, w, ds, bcs, 33)
+    np.testing.assert_array_equal(got, U)
+    assert (t, n) == (to, no)
