@@ -48,35 +48,6 @@ __global__ void arm_kernel(dg::Scalars *sc, double t_end, long long max_steps_ab
     sc->active = !((sc->t > sc->t_end) || (sc->steps >= sc->max_steps));
 }
 
-int launch_one_step(dg_ctx *c)
-{
-    const void *in[5];
-    void *out[5];
-    for (int f = 0; f < 5; ++f) { in[f] = c->buf[c->cur][f]; out[f] = c->buf[1 - c->cur][f]; }
-    int rc = dg::launch_step(c->prec, c->fast, c->tile_y, c->args, in, out, c->sc, c->partials, c->stream);
-    if (rc) return fail(c, DG_ERR_DEVICE, std::string("step launch: ") + hipGetErrorString(hipGetLastError()));
-    rc = dg::launch_finalize(c->prec, c->args, c->sc, c->partials, c->args.nblocks, c->cfg.cfl,
-                             c->cfg.dt_nan_fallback, 1, c->stream);
-    if (rc) return fail(c, DG_ERR_DEVICE, "finalize launch");
-    c->cur = 1 - c->cur;
-    c->sums_stale = true;
-    return DG_OK;
-}
-
-// mode 0: dt and sums of a freshly set state; 2: sums only
-int reduce_current(dg_ctx *c, int mode)
-{
-    const void *u[5];
-    for (int f = 0; f < 5; ++f) u[f] = c->buf[c->cur][f];
-    if (dg::launch_reduce(c->prec, c->fast, c->args, u, c->partials, c->nparts_reduce, c->stream))
-        return fail(c, DG_ERR_DEVICE, "reduce launch");
-    if (dg::launch_finalize(c->prec, c->args, c->sc, c->partials, c->nparts_reduce, c->cfg.cfl,
-                            c->cfg.dt_nan_fallback, mode, c->stream))
-        return fail(c, DG_ERR_DEVICE, "finalize launch");
-    c->sums_stale = false;
-    return DG_OK;
-}
-
 int read_scalars(dg_ctx *c)
 {
     HIPCHK(c, hipMemcpyAsync(c->sc_host, c->sc, sizeof(dg::Scalars), hipMemcpyDeviceToHost, c->stream));
@@ -231,6 +202,36 @@ int reserve_partials(dg_ctx *c)
     return DG_OK;
 }
 
+// One step (the step kernel, then dt / clock) on the context's stream.
+int launch_one_step(dg_ctx *c)
+{
+    const void *in[5];
+    void *out[5];
+    for (int f = 0; f < 5; ++f) { in[f] = c->buf[c->cur][f]; out[f] = c->buf[1 - c->cur][f]; }
+    int rc = dg::launch_step(c->prec, c->fast, c->tile_y, c->args, in, out, c->sc, c->partials, c->stream);
+    if (rc) return fail(c, DG_ERR_DEVICE, std::string("step launch: ") + hipGetErrorString(hipGetLastError()));
+    rc = dg::launch_finalize(c->prec, c->args, c->sc, c->partials, c->args.nblocks, c->cfg.cfl,
+                             c->cfg.dt_nan_fallback, 1, c->stream);
+    if (rc) return fail(c, DG_ERR_DEVICE, "finalize launch");
+    c->cur = 1 - c->cur;
+    c->sums_stale = true;
+    return DG_OK;
+}
+
+// mode 0: dt and sums of a freshly set state; 2: sums only
+int reduce_current(dg_ctx *c, int mode)
+{
+    const void *u[5];
+    for (int f = 0; f < 5; ++f) u[f] = c->buf[c->cur][f];
+    if (dg::launch_reduce(c->prec, c->fast, c->args, u, c->partials, c->nparts_reduce, c->stream))
+        return fail(c, DG_ERR_DEVICE, "reduce launch");
+    if (dg::launch_finalize(c->prec, c->args, c->sc, c->partials, c->nparts_reduce, c->cfg.cfl,
+                            c->cfg.dt_nan_fallback, mode, c->stream))
+        return fail(c, DG_ERR_DEVICE, "finalize launch");
+    c->sums_stale = false;
+    return DG_OK;
+}
+
 // Contexts of an ensemble share the ensemble's stream (shared_stream != null).
 int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out)
 {
@@ -353,7 +354,7 @@ int dg_set_state(dg_ctx *c, const void *const f[5], int kind)
         for (int q = 0; q < 5; ++q) u[q] = c->buf[c->cur][q];
         if (dg::launch_prim_to_cons(c->prec, c->args, u, c->stream)) return fail(c, DG_ERR_DEVICE, "prim_to_cons");
     }
-    int rc = reduce_current(c, 0);
+    int rc = dg::reduce_current(c, 0);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_state = true;
@@ -401,7 +402,7 @@ int dg_step(dg_ctx *c, double *dt_used)
     if (rc) return rc;
     const double dt = c->sc_host->dt;
     if (!c->cfg.dt_nan_fallback && !std::isfinite(dt)) return fail(c, DG_ERR_NONFINITE, "dt is not finite");
-    rc = launch_one_step(c);
+    rc = dg::launch_one_step(c);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (dt_used) *dt_used = dt;
@@ -426,7 +427,7 @@ static int ensure_graph(dg_ctx *c)
     const int cur0 = c->cur;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = DG_OK;
-    for (int s = 0; s < kGraphSteps && rc == DG_OK; ++s) rc = launch_one_step(c);
+    for (int s = 0; s < kGraphSteps && rc == DG_OK; ++s) rc = dg::launch_one_step(c);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->cur = cur0;  // capture launched nothing
     if (rc) { if (g) hipGraphDestroy(g); return rc; }
@@ -457,7 +458,7 @@ int dg_run(dg_ctx *c, int64_t max_steps, double t_end, double *t_out, int64_t *s
             HIPCHK(c, hipGraphLaunch(c->graph[c->cur], c->stream));
             left -= c->graph_steps;
         } else {
-            rc = launch_one_step(c);
+            rc = dg::launch_one_step(c);
             if (rc) return rc;
             left -= 1;
         }
@@ -480,7 +481,7 @@ int dg_conservation_sums(dg_ctx *c, double out[5])
     if (!c || !out) return DG_ERR_INVALID;
     if (!c->have_state) return fail(c, DG_ERR_STATE, "no state set");
     int rc = DG_OK;
-    if (c->sums_stale && (rc = reduce_current(c, 2))) return rc;
+    if (c->sums_stale && (rc = dg::reduce_current(c, 2))) return rc;
     if ((rc = read_scalars(c))) return rc;
     for (int q = 0; q < 5; ++q) out[q] = c->sc_host->sums[q];
     return DG_OK;

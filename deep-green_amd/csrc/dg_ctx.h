@@ -37,4 +37,8 @@ namespace dg {
 int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out);
 int set_tiling(dg_ctx *c, int tile_y, int chunk_x);
 int reserve_partials(dg_ctx *c);
+// One step (step kernel, then dt and clock: finalize mode 1); flips c->cur.
+int launch_one_step(dg_ctx *c);
+// Reduction of the current state: mode 0 dt and sums of a fresh state, 2 sums only.
+int reduce_current(dg_ctx *c, int mode);
 }

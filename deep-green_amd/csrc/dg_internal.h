@@ -73,6 +73,17 @@ int launch_ens_step(int prec, int fast, int tile_y, int limiter, const EnsInstan
                     int nitems, int grid, int parity, double *partials, hipStream_t s);
 int launch_ens_finalize(int prec, const EnsInstance *inst, int ninst, const double *partials, int mode,
                         hipStream_t s);
+// Learned flux correction (FP32 periodic grids): flags cells whose corrected
+// state is not positive, applies the masked correction in place and writes
+// reduce_blocks(a) wave-speed partials for the finalize (mode 1).
+int launch_corr(int fast, const StepArgs &a, void *const u[5], const float *G, const float scale[5], const Scalars *sc,
+                unsigned char *flags, double *partials, hipStream_t s);
+// FP32: coarse = 2x restriction of fine (sum of the 8 children).
+int launch_restrict(const StepArgs &fine, const void *const f[5], const StepArgs &coarse, void *const c[5], hipStream_t s);
+// FP32: r = (ut - up) / (dt ds^2).
+int launch_residual(long long n3, const void *const ut[5], const void *const up[5], double dt, double ds,
+                    void *const r[5], hipStream_t s);
+int launch_set_dt(Scalars *sc, double dt, hipStream_t s);
 // Resident workgroups of the ensemble step kernel per CU (-1 on error).
 int ens_blocks_per_cu(int prec, int fast, int tile_y, int limiter);
 int step_lds_bytes(int prec, int tile_y);

@@ -966,6 +966,122 @@ cons_to_prim_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> W
     }
 }
 
+// ------------------------------------------ learned flux correction --
+// The network's output G (fp32, 16 channels per cell, channel 5d+q = the
+// correction of flux component q on the face c + e_d/2, in units of
+// out_scale[q]) corrects the plain update of a periodic FP32 grid:
+//   U[c] += -dt ds^2 out_scale[q] sum_d (m(c, c+e_d) G[c][5d+q] - m(c-e_d, c) G[c-e_d][5d+q]),
+// the face flux correction telescopes, so the totals are unchanged.  A cell
+// whose corrected state would have rho <= 0 or p <= 0 is flagged; every face
+// of a flagged cell keeps the plain flux (m = 0), so each face still carries
+// one flux for both of its cells.
+struct CorrScale { float s[5]; };
+
+__device__ __forceinline__ long long wrap_index(int x, int y, int z, const StepArgs &A)
+{
+    x = x < 0 ? x + A.n[0] : (x >= A.n[0] ? x - A.n[0] : x);
+    y = y < 0 ? y + A.n[1] : (y >= A.n[1] ? y - A.n[1] : y);
+    z = z < 0 ? z + A.n[2] : (z >= A.n[2] ? z - A.n[2] : z);
+    return ((long long)x * A.n[1] + y) * A.n[2] + z;
+}
+
+// Corrected increment of cell (x, y, z); flags == nullptr: all faces corrected.
+__device__ __forceinline__ void corr_delta(const StepArgs &A, const float *__restrict__ G, const CorrScale &S,
+                                           float coef, const unsigned char *__restrict__ flags, int x, int y, int z,
+                                           float d[5])
+{
+    const long long c = wrap_index(x, y, z, A);
+    const long long lo[3] = {wrap_index(x - 1, y, z, A), wrap_index(x, y - 1, z, A), wrap_index(x, y, z - 1, A)};
+    const long long hi[3] = {wrap_index(x + 1, y, z, A), wrap_index(x, y + 1, z, A), wrap_index(x, y, z + 1, A)};
+    const bool fc = flags && flags[c];
+    float mh[3], ml[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        mh[a] = (fc || (flags && flags[hi[a]])) ? 0.0f : 1.0f;
+        ml[a] = (fc || (flags && flags[lo[a]])) ? 0.0f : 1.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        float div = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) div += mh[a] * G[c * 16 + 5 * a + q] - ml[a] * G[lo[a] * 16 + 5 * a + q];
+        d[q] = coef * S.s[q] * div;
+    }
+}
+
+template <bool FAST>
+__global__ void __launch_bounds__(256)
+corr_flag_kernel(const StepArgs A, const Fields<const float> U, const float *__restrict__ G, const CorrScale S,
+                 const Scalars *__restrict__ sc, unsigned char *__restrict__ flags)
+{
+    const float ds = float(A.ds), V = ds * ds * ds, gm1 = float(A.gamma) - 1.0f;
+    const float coef = sc->active ? -float(sc->dt) * ds * ds : 0.0f;   // an inactive step changes nothing
+    const long long n3 = (long long)A.n[0] * A.n[1] * A.n[2];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(i % A.n[2]), y = (int)((i / A.n[2]) % A.n[1]), x = (int)(i / ((long long)A.n[1] * A.n[2]));
+        float d[5], u[5], w[5];
+        corr_delta(A, G, S, coef, nullptr, x, y, z, d);
+        for (int q = 0; q < 5; ++q) u[q] = U.f[q][i] + d[q];
+        to_prim<float, FAST>(u, V, 1.0f / V, gm1, w);
+        flags[i] = (w[0] > 0.0f && w[4] > 0.0f) ? 0 : 1;
+    }
+}
+
+template <bool FAST>
+__global__ void __launch_bounds__(256)
+corr_apply_kernel(const StepArgs A, const Fields<float> U, const float *__restrict__ G, const CorrScale S,
+                  const Scalars *__restrict__ sc, const unsigned char *__restrict__ flags, double *__restrict__ partials)
+{
+    __shared__ double red[4 * 8];
+    const float ds = float(A.ds), V = ds * ds * ds, gm1 = float(A.gamma) - 1.0f, gamma = float(A.gamma);
+    const float coef = sc->active ? -float(sc->dt) * ds * ds : 0.0f;
+    const long long n3 = (long long)A.n[0] * A.n[1] * A.n[2];
+    float maxden = 0.0f;
+    int nanf = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(i % A.n[2]), y = (int)((i / A.n[2]) % A.n[1]), x = (int)(i / ((long long)A.n[1] * A.n[2]));
+        float d[5], u[5], w[5];
+        corr_delta(A, G, S, coef, flags, x, y, z, d);
+        for (int q = 0; q < 5; ++q) {
+            u[q] = U.f[q][i] + d[q];
+            U.f[q][i] = u[q];
+        }
+        to_prim<float, FAST>(u, V, 1.0f / V, gm1, w);
+        const float den = wave_den<float, FAST>(w, gamma);
+        if (den != den) nanf = 1; else maxden = fmaxf(maxden, den);
+    }
+    const double nosums[5] = {0, 0, 0, 0, 0};
+    block_partials<256>((double)maxden, nanf, nosums, red, partials, blockIdx.x, threadIdx.x);
+}
+
+// 2x restriction: coarse[I] = sum of the 8 children (U is extensive).
+__global__ void __launch_bounds__(256)
+restrict_kernel(const StepArgs Af, const Fields<const float> F, const StepArgs Ac, const Fields<float> C)
+{
+    const long long n3 = (long long)Ac.n[0] * Ac.n[1] * Ac.n[2];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(i % Ac.n[2]), y = (int)((i / Ac.n[2]) % Ac.n[1]), x = (int)(i / ((long long)Ac.n[1] * Ac.n[2]));
+        for (int q = 0; q < 5; ++q) {
+            float s = 0.0f;
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b)
+                    for (int c = 0; c < 2; ++c)
+                        s += F.f[q][((long long)(2 * x + a) * Af.n[1] + (2 * y + b)) * Af.n[2] + (2 * z + c)];
+            C.f[q][i] = s;
+        }
+    }
+}
+
+// R = (Ut - Up) / (dt ds^2): the flux divergence the coarse update misses.
+__global__ void __launch_bounds__(256)
+residual_kernel(long long n3, const Fields<const float> Ut, const Fields<const float> Up, float inv, const Fields<float> R)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long long)gridDim.x * blockDim.x)
+        for (int q = 0; q < 5; ++q) R.f[q][i] = (Ut.f[q][i] - Up.f[q][i]) * inv;
+}
+
+__global__ void set_dt_kernel(Scalars *sc, double dt) { sc->dt = dt; }
+
 // ---------------------------------------------------------------- launch --
 template <typename T> static Fields<const T> cfields(const void *const p[5])
 {
@@ -1100,6 +1216,52 @@ int launch_ens_finalize(int prec, const EnsInstance *inst, int ninst, const doub
         hipLaunchKernelGGL((ens_finalize_kernel<float, false>), dim3(ninst), dim3(256), 0, s, inst, partials, mode);
     else
         hipLaunchKernelGGL((ens_finalize_kernel<float, true>), dim3(ninst), dim3(256), 0, s, inst, partials, mode);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+static int cells_grid(long long n3)
+{
+    long long b = (n3 + 255) / 256;
+    return (int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
+
+int launch_corr(int fast, const StepArgs &a, void *const u[5], const float *G, const float scale[5], const Scalars *sc,
+                unsigned char *flags, double *partials, hipStream_t s)
+{
+    CorrScale S;
+    for (int q = 0; q < 5; ++q) S.s[q] = scale[q];
+    const int nb = reduce_blocks(a);
+    const long long n3 = (long long)a.n[0] * a.n[1] * a.n[2];
+    if (fast) {
+        hipLaunchKernelGGL((corr_flag_kernel<true>), dim3(cells_grid(n3)), dim3(256), 0, s, a, cfields<float>(u), G, S, sc, flags);
+        hipLaunchKernelGGL((corr_apply_kernel<true>), dim3(nb), dim3(256), 0, s, a, mfields<float>(u), G, S, sc, flags, partials);
+    } else {
+        hipLaunchKernelGGL((corr_flag_kernel<false>), dim3(cells_grid(n3)), dim3(256), 0, s, a, cfields<float>(u), G, S, sc, flags);
+        hipLaunchKernelGGL((corr_apply_kernel<false>), dim3(nb), dim3(256), 0, s, a, mfields<float>(u), G, S, sc, flags, partials);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_restrict(const StepArgs &fine, const void *const f[5], const StepArgs &coarse, void *const c[5], hipStream_t s)
+{
+    const long long n3 = (long long)coarse.n[0] * coarse.n[1] * coarse.n[2];
+    hipLaunchKernelGGL(restrict_kernel, dim3(cells_grid(n3)), dim3(256), 0, s, fine, cfields<float>(f), coarse,
+                       mfields<float>(c));
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_residual(long long n3, const void *const ut[5], const void *const up[5], double dt, double ds,
+                    void *const r[5], hipStream_t s)
+{
+    const float inv = (float)(1.0 / (dt * ds * ds));
+    hipLaunchKernelGGL(residual_kernel, dim3(cells_grid(n3)), dim3(256), 0, s, n3, cfields<float>(ut), cfields<float>(up),
+                       inv, mfields<float>(r));
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_set_dt(Scalars *sc, double dt, hipStream_t s)
+{
+    hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, s, sc, dt);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

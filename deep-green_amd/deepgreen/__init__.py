@@ -37,7 +37,13 @@ EXPORTS = ("dg_config_default", "dg_create", "dg_destroy", "dg_last_error", "dg_
            "dg_device_info",
            "dg_ensemble_create", "dg_ensemble_destroy", "dg_ensemble_last_error", "dg_ensemble_set_state",
            "dg_ensemble_get_state", "dg_ensemble_run", "dg_ensemble_clock", "dg_ensemble_conservation_sums",
-           "dg_ensemble_time_steps", "dg_ensemble_retile", "dg_ensemble_info")
+           "dg_ensemble_time_steps", "dg_ensemble_retile", "dg_ensemble_info",
+           "dg_step_dt", "dg_restrict",
+           "dg_cnn_create", "dg_cnn_destroy", "dg_cnn_last_error", "dg_cnn_param_count", "dg_cnn_get_params",
+           "dg_cnn_set_params", "dg_cnn_get_normalization", "dg_cnn_set_normalization", "dg_cnn_forward",
+           "dg_cnn_backward", "dg_cnn_train_batch", "dg_cnn_adam_step", "dg_cnn_add_sample", "dg_cnn_sample_count",
+           "dg_cnn_fit_normalization", "dg_cnn_train", "dg_learned_step", "dg_learned_run", "dg_learned_flagged",
+           "dg_learned_correction", "dg_learned_step_given")
 
 
 class DgConfig(ctypes.Structure):
@@ -107,9 +113,36 @@ def load_library(path=None):
     L.dg_ensemble_retile.argtypes = [vp, i32, P(ctypes.c_int32), i32]
     L.dg_ensemble_info.argtypes = [vp, P(i64), P(i32), P(i32), P(i32)]
     L.dg_device_info.argtypes = [i32, ctypes.c_char_p, i32, P(i32)]
+    fp = P(ctypes.c_float)
+    L.dg_step_dt.argtypes = [vp, d]
+    L.dg_restrict.argtypes = [vp, vp]
+    L.dg_cnn_create.argtypes = [i32, ctypes.c_uint64, P(vp)]
+    L.dg_cnn_destroy.argtypes = [vp]
+    L.dg_cnn_destroy.restype = None
+    L.dg_cnn_last_error.argtypes = [vp]
+    L.dg_cnn_last_error.restype = ctypes.c_char_p
+    L.dg_cnn_param_count.argtypes = []
+    L.dg_cnn_param_count.restype = ctypes.c_int64
+    L.dg_cnn_get_params.argtypes = [vp, fp]
+    L.dg_cnn_set_params.argtypes = [vp, fp]
+    L.dg_cnn_get_normalization.argtypes = [vp, fp]
+    L.dg_cnn_set_normalization.argtypes = [vp, fp]
+    L.dg_cnn_forward.argtypes = [vp, i32, P(ctypes.c_int32), fp, fp]
+    L.dg_cnn_backward.argtypes = [vp, i32, P(ctypes.c_int32), fp, fp, fp]
+    L.dg_cnn_train_batch.argtypes = [vp, i32, i32, fp, fp, P(d), fp]
+    L.dg_cnn_adam_step.argtypes = [vp, fp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+    L.dg_cnn_add_sample.argtypes = [vp, vp, vp]
+    L.dg_cnn_sample_count.argtypes = [vp]
+    L.dg_cnn_fit_normalization.argtypes = [vp]
+    L.dg_cnn_train.argtypes = [vp, i32, i32, i32, ctypes.c_float, ctypes.c_uint64, fp]
+    L.dg_learned_step.argtypes = [vp, vp, P(d)]
+    L.dg_learned_run.argtypes = [vp, vp, i64, d, P(d), P(i64)]
+    L.dg_learned_flagged.argtypes = [vp, P(i64)]
+    L.dg_learned_correction.argtypes = [vp, vp, fp]
+    L.dg_learned_step_given.argtypes = [vp, vp, fp, P(d)]
     for name in EXPORTS:
         if name not in ("dg_config_default", "dg_destroy", "dg_last_error", "dg_ensemble_destroy",
-                        "dg_ensemble_last_error"):
+                        "dg_ensemble_last_error", "dg_cnn_destroy", "dg_cnn_last_error", "dg_cnn_param_count"):
             getattr(L, name).restype = ctypes.c_int
     if path is None:
         _lib = L
@@ -231,6 +264,18 @@ class Solver:
         dt = ctypes.c_double()
         self._check(self._L.dg_step(self._h, ctypes.byref(dt)), "dg_step")
         return dt.value
+
+    def step_dt(self, dt):
+        """One step with the given dt (the next step uses the CFL dt again)."""
+        self._check(self._L.dg_step_dt(self._h, float(dt)), "dg_step_dt")
+
+    def restrict_from(self, fine):
+        """This (coarse) grid's state := the 2x restriction of `fine`'s state."""
+        self._check(self._L.dg_restrict(fine._h, self._h), "dg_restrict")
+
+    @property
+    def handle(self):
+        return self._h
 
     def run(self, max_steps, t_end=math.inf):
         t = ctypes.c_double()
@@ -372,9 +417,14 @@ class Ensemble:
                     "dg_ensemble_time_steps")
         return a.value, b.value
 
-    def set_grid(self, grid):
-        """Persistent workgroups of the step launch (0: device maximum)."""
-        self._check(self._L.dg_ensemble_set_grid(self._h, int(grid)), "dg_ensemble_retile")
+    def retile(self, tile_y=0, chunk_x=None, grid=0):
+        """New tiling (results unchanged): tile_y for all (0 keeps it),
+        chunk_x per instance (None keeps them, 0 automatic), grid persistent
+        workgroups (0: device maximum)."""
+        arr = None
+        if chunk_x is not None:
+            arr = (ctypes.c_int32 * len(self.sims))(*[int(c) for c in chunk_x])
+        self._check(self._L.dg_ensemble_retile(self._h, int(tile_y), arr, int(grid)), "dg_ensemble_retile")
 
     def info(self):
         cells, items, grid, ty = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -187,6 +187,80 @@ int dg_ensemble_retile(dg_ensemble *ens, int tile_y, const int32_t *chunk_x, int
  * workgroups, rows per workgroup. */
 int dg_ensemble_info(dg_ensemble *ens, int64_t *cells, int *items, int *grid, int *tile_y);
 
+/* One step with a given dt instead of the CFL dt (the dt of the step after
+ * is the CFL dt again).  No reference counterpart (compute_step ignores its
+ * incoming dt, physics.rs:595-599); used to generate training data. */
+int dg_step_dt(dg_ctx *ctx, double dt);
+
+/* FP32 grids: the state of `coarse` becomes the 2x restriction of the state of
+ * `fine` (n_fine = 2 n_coarse per axis; each coarse cell holds the sum of its
+ * 8 children, the conserved values being extensive, physics.rs:143-175). */
+int dg_restrict(dg_ctx *fine, dg_ctx *coarse);
+
+/* ------------------------------------------------ learned flux correction --
+ * A 3D CNN 5 -> 32 -> 32 -> 15 (3x3x3 kernels, GELU after layers 1 and 2;
+ * bf16 MFMA with fp32 accumulation; fp32 master weights and Adam) maps the
+ * normalized primitive state of a periodic FP32 grid to corrections of the
+ * five flux components on the three upper faces of every cell.  A learned
+ * step adds them to the plain step's face fluxes, so the totals are conserved
+ * to rounding; every face of a cell whose corrected density or pressure would
+ * not be positive keeps the plain flux.  No reference counterpart.
+ *
+ * Parameters (fp32, dg_cnn_param_count() values): W1 [27][5][32], b1 [32],
+ * W2 [27][32][32], b2 [32], W3 [27][32][15], b3 [15]; tap = (dx*3 + dy)*3 + dz.
+ * Normalization (15 floats): input shift[5] and scale[5] of (rho, vx, vy, vz,
+ * p), output scale[5] of the five flux components. */
+typedef struct dg_cnn dg_cnn;
+
+int dg_cnn_create(int device, uint64_t seed, dg_cnn **out);
+void dg_cnn_destroy(dg_cnn *m);
+const char *dg_cnn_last_error(const dg_cnn *m);
+int64_t dg_cnn_param_count(void);
+int dg_cnn_get_params(dg_cnn *m, float *params);
+int dg_cnn_set_params(dg_cnn *m, const float *params);
+int dg_cnn_get_normalization(dg_cnn *m, float norm[15]);
+int dg_cnn_set_normalization(dg_cnn *m, const float norm[15]);
+
+/* Network output y [batch][dims-6 cells][15] for inputs x [batch][dims cells][5]
+ * (host fp32, channels last, already normalized; valid convolutions). */
+int dg_cnn_forward(dg_cnn *m, int batch, const int32_t dims[3], const float *x, float *y);
+/* Parameter gradients of sum(y * dy) (dy [batch][dims-6 cells][15]); grads may
+ * be NULL (kept on the device for dg_cnn_adam_step). */
+int dg_cnn_backward(dg_cnn *m, int batch, const int32_t dims[3], const float *x, const float *dy, float *grads);
+/* One training batch: inputs x [batch][(L+7)^3][5] (normalized), targets
+ * [batch][L^3][5] (normalized); loss = mean over cells and components of
+ * (-div G - target)^2 with G on the (L+1)^3 faces; gradients kept on the device. */
+int dg_cnn_train_batch(dg_cnn *m, int batch, int L, const float *x, const float *target, double *loss, float *grads);
+/* Adam (fp32 master weights) with grads from the host, or the last computed
+ * gradients when grads is NULL. */
+int dg_cnn_adam_step(dg_cnn *m, const float *grads, float lr, float beta1, float beta2, float eps);
+
+/* Training data from the solver: restricts `fine` into `coarse` (U^n), steps
+ * the coarse grid once and the fine grid twice with a common dt (the smaller
+ * of the coarse CFL dt and twice the fine one), and stores U^n and the missed
+ * flux divergence R = (restricted fine - coarse) / (dt ds^2).  Both grids
+ * FP32 and periodic; the fine state advances by two steps. */
+int dg_cnn_add_sample(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse);
+int dg_cnn_sample_count(dg_cnn *m);
+/* Normalization from the samples (mean/std of the primitives, RMS of R). */
+int dg_cnn_fit_normalization(dg_cnn *m);
+/* steps of Adam (beta 0.9/0.999, eps 1e-8) on random periodic crops of the
+ * samples (batch crops of L^3 cells); losses[steps] (may be NULL). */
+int dg_cnn_train(dg_cnn *m, int steps, int batch, int L, float lr, uint64_t seed, float *losses);
+
+/* dg_step / dg_run with the learned correction (ctx: FP32, periodic). */
+int dg_learned_step(dg_ctx *ctx, dg_cnn *m, double *dt_used);
+int dg_learned_run(dg_ctx *ctx, dg_cnn *m, int64_t max_steps, double t_end, double *t_out, int64_t *steps_out);
+/* A learned step with the correction G [n0*n1*n2][16] given by the host
+ * (normalized units, as dg_learned_correction returns) instead of the
+ * network's: for tests of the correction, the fallback and conservation. */
+int dg_learned_step_given(dg_cnn *m, dg_ctx *ctx, const float *G, double *dt_used);
+/* Cells that kept the plain flux in the last learned step. */
+int dg_learned_flagged(dg_cnn *m, int64_t *count);
+/* Network output for the current state of ctx: G [n0*n1*n2][16] (channel
+ * 5*axis + component, channel 15 unused), normalized units. */
+int dg_learned_correction(dg_cnn *m, dg_ctx *ctx, float *G);
+
 #ifdef __cplusplus
 }
 #endif

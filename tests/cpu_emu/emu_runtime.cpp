@@ -13,6 +13,7 @@ emu_idx blockDim, gridDim;
 
 namespace emu {
 double xbuf[1024];
+float mfa[1024][8], mfb[1024][8];
 bool capturing = false;
 std::vector<std::function<void()>> captured;
 

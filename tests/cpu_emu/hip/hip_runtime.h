@@ -83,7 +83,7 @@ typedef void *hipEvent_t;
 typedef void *hipGraph_t;
 typedef void *hipGraphExec_t;
 enum { hipSuccess = 0, hipErrorUnknown = 999 };
-enum { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipStreamNonBlocking = 1,
+enum { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3, hipStreamNonBlocking = 1,
        hipHostMallocDefault = 0, hipStreamCaptureModeThreadLocal = 0,
        hipDeviceAttributeMultiprocessorCount = 16 };
 typedef int hipMemcpyKind;
@@ -122,6 +122,7 @@ inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, int, hipStrea
     std::memcpy(d, s, n);
     return hipSuccess;
 }
+inline hipError_t hipMemset(void *p, int v, size_t n) { std::memset(p, v, n); return hipSuccess; }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, int) { *s = (void *)1; return hipSuccess; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
@@ -162,6 +163,49 @@ template <typename F> inline hipError_t hipOccupancyMaxActiveBlocksPerMultiproce
 {
     *n = 1;
     return hipSuccess;
+}
+// Matrix-core operations, computed from the operands of all 64 lanes of the
+// wave with the documented lane layouts (see dg_cnn.hip); sums in fp32.
+typedef __bf16 emu_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float emu_f32x16 __attribute__((ext_vector_type(16)));
+typedef float emu_f32x4 __attribute__((ext_vector_type(4)));
+namespace emu {
+extern float mfa[1024][8], mfb[1024][8];
+inline void mf_publish(const emu_bf16x8 &a, const emu_bf16x8 &b)
+{
+    const int tid = (int)threadIdx.x;
+    sync();
+    for (int i = 0; i < 8; ++i) { mfa[tid][i] = (float)a[i]; mfb[tid][i] = (float)b[i]; }
+    sync();
+}
+}  // namespace emu
+inline emu_f32x16 __builtin_amdgcn_mfma_f32_32x32x16_bf16(emu_bf16x8 a, emu_bf16x8 b, emu_f32x16 c, int, int, int)
+{
+    emu::mf_publish(a, b);
+    const int tid = (int)threadIdx.x, l = tid & 63, w = tid & ~63, j = l % 32;
+    emu_f32x16 d = c;
+    for (int i = 0; i < 16; ++i) {
+        const int r = 8 * (i / 4) + 4 * (l / 32) + (i % 4);
+        float s = 0.0f;
+        for (int k = 0; k < 16; ++k) s += emu::mfa[w + r + 32 * (k / 8)][k % 8] * emu::mfb[w + j + 32 * (k / 8)][k % 8];
+        d[i] = c[i] + s;
+    }
+    emu::sync();
+    return d;
+}
+inline emu_f32x4 __builtin_amdgcn_mfma_f32_16x16x32_bf16(emu_bf16x8 a, emu_bf16x8 b, emu_f32x4 c, int, int, int)
+{
+    emu::mf_publish(a, b);
+    const int tid = (int)threadIdx.x, l = tid & 63, w = tid & ~63, j = l % 16;
+    emu_f32x4 d = c;
+    for (int i = 0; i < 4; ++i) {
+        const int r = 4 * (l / 16) + i;
+        float s = 0.0f;
+        for (int k = 0; k < 32; ++k) s += emu::mfa[w + r + 16 * (k / 8)][k % 8] * emu::mfb[w + j + 16 * (k / 8)][k % 8];
+        d[i] = c[i] + s;
+    }
+    emu::sync();
+    return d;
 }
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
