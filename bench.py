@@ -143,12 +143,14 @@ def bench_ensemble(args):
                                log=lambda s: print(s, file=sys.stderr, flush=True))
     tune_s = time.perf_counter() - t0
     sims, states = ensemble_workload(args.members, args.seed, math=args.math, choice=choice)
-    ens = dg.Ensemble(sims, grid=autotune.grid_for(choice["blocks_per_cu"]))
+    ens = dg.Ensemble(sims)
+    autotune.apply(ens, choice)
     for i, w in states():
         ens.set_state(i, w)
     info = ens.info()
     print(f"[bench] ensemble of {len(sims)} ({dict(zip(sizes, counts))}), {info}, tiles {choice['tile_y']} "
-          f"{choice['chunk_x']} blocks/CU {choice['blocks_per_cu']}", file=sys.stderr, flush=True)
+          f"{choice['chunk_x']} waves {choice.get('waves', 4)} blocks/CU {choice['blocks_per_cu']}",
+          file=sys.stderr, flush=True)
     ens.run(args.warmup)
     ens.clock(0)
     t0 = time.perf_counter()
@@ -170,7 +172,7 @@ def bench_ensemble(args):
                    "sizes": dict(zip(map(str, sizes), counts)), "seed": args.seed, "boundaries": "outflow",
                    "gamma": 1.4, "cfl": 0.4, "math": args.math, "limiter": "minmod",
                    "pct_hbm_peak": round(100.0 * gcells * 1e9 * 40 / (HBM_PEAK_GBS * 1e9), 2),
-                   "tile_y": choice["tile_y"], "chunk_x": choice["chunk_x"],
+                   "tile_y": choice["tile_y"], "waves": choice.get("waves", 4), "chunk_x": choice["chunk_x"],
                    "blocks_per_cu": choice["blocks_per_cu"], "work_items": info["items"], "grid": info["grid"],
                    "autotune_s": round(tune_s, 2), "tuned_in_run": tuned_now, "state_finite": finite},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -180,12 +182,57 @@ def bench_ensemble(args):
     }
 
 
+def bench_learned(args):
+    """Config: learned flux correction at 128^3 (FP32, minmod, periodic).
+    Training data from plain 256^3 runs (seeded random ICs) restricted to
+    128^3; Adam lr 1e-3, batch 8 of 32^3 crops; evaluation against the plain
+    512^3 solution restricted to 128^3, all runs from restrictions of the
+    same 512^3 initial condition, to the same final time."""
+    from deepgreen.learned import FluxCNN, evaluate, make_training_set
+
+    def log(s):
+        print(s, file=sys.stderr, flush=True)
+
+    m = FluxCNN(seed=args.seed)
+    t0 = time.perf_counter()
+    make_training_set(m, seeds=range(1, 1 + args.train_ics), n_fine=2 * args.n_learned, t_end=args.t_end,
+                      every=args.sample_every, log=log)
+    gen_s = time.perf_counter() - t0
+    m.fit_normalization()
+    t0 = time.perf_counter()
+    losses = m.train(args.train_steps, batch=8, crop=32, lr=1e-3, seed=args.seed)
+    train_s = time.perf_counter() - t0
+    log(f"[learned] {m.samples} samples in {gen_s:.1f} s; {args.train_steps} Adam steps in {train_s:.1f} s, "
+        f"loss {losses[:5].mean():.4g} -> {losses[-50:].mean():.4g}")
+    res = evaluate(m, seed=1000 + args.seed, n_truth=4 * args.n_learned, T=args.t_end, log=log)
+    e, r = res["l1_rho"], res["runs"]
+    speedup = r["plain_half"]["wall_s"] / r["learned_quarter"]["wall_s"]
+    matched = e["learned_quarter"] <= e["plain_half"]
+    n = args.n_learned
+    return {
+        "metric": "error-matched speedup learned vs plain", "unit": "x", "value": speedup, "dtype": "f32",
+        "s_per_step": r["learned_quarter"]["wall_s"] / max(r["learned_quarter"]["steps"], 1),
+        "config": {"workload": f"learned{n}_fp32", "error_matched": matched,
+                   "l1_rho": e, "runs": r, "T": args.t_end, "train_ics": args.train_ics,
+                   "samples": m.samples, "train_steps": args.train_steps, "batch": 8, "crop": 32, "lr": 1e-3,
+                   "loss_first": float(losses[:5].mean()), "loss_last": float(losses[-50:].mean()),
+                   "data_gen_s": round(gen_s, 2), "train_s": round(train_s, 2),
+                   "mass_learned_vs_plain": res["mass_learned_vs_plain"]},
+        "roofline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (khi: 200, ensemble: 100)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="khi", choices=["khi", "ensemble"])
+    ap.add_argument("--workload", default="khi", choices=["khi", "ensemble", "learned"])
+    ap.add_argument("--n-learned", type=int, default=128)
+    ap.add_argument("--train-ics", type=int, default=4)
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--sample-every", type=int, default=8)
+    ap.add_argument("--t-end", type=float, default=0.03)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
@@ -214,14 +261,19 @@ def main():
     if args.steps is None:
         args.steps = 100 if args.workload == "ensemble" else 200
 
-    r = bench_ensemble(args) if args.workload == "ensemble" else bench_khi(args)
+    if args.workload == "ensemble":
+        r = bench_ensemble(args)
+    elif args.workload == "learned":
+        r = bench_learned(args)
+    else:
+        r = bench_khi(args)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_sample)
     out = {
-        "metric": "Gcell-updates/s + % HBM peak",
+        "metric": r.get("metric", "Gcell-updates/s + % HBM peak"),
         "value": round(r["value"], 3),
-        "unit": "Gcell-updates/s",
+        "unit": r.get("unit", "Gcell-updates/s"),
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,

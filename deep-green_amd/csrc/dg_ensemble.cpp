@@ -5,11 +5,13 @@
 // reduction slots) created on the ensemble's stream, so state upload and
 // download, the fresh-state dt and the conservation sums are the single-grid
 // code.  A step of the ensemble is two launches:
-//   ens_step_kernel      persistent workgroups striding over the (instance,
-//                        tile) work list, heaviest tiles first; each tile
-//                        writes one partial slot of its instance;
+//   ens_step_kernel      persistent workgroups taking (instance, tile) items
+//                        from the work list (heaviest first) through an
+//                        atomic counter; each tile writes one partial slot of
+//                        its instance;
 //   ens_finalize_kernel  one workgroup per instance: its partial slots -> its
-//                        next dt and run clock (the finalize of dg_run).
+//                        next dt and run clock (the finalize of dg_run); it
+//                        also resets the work counter for the next step.
 // The per-tile arithmetic is the single-grid step's (same device function),
 // the max wave speed is an order-free max, so each instance's trajectory is
 // bitwise the one dg_run produces, whatever the tile shape or the order of the
@@ -37,11 +39,13 @@ const int kEnsGraphSteps = 16;  // even: the buffer parity is unchanged by a rep
 struct dg_ensemble {
     std::vector<dg_ctx *> inst;
     int prec = 1, fast = 0, tile_y = 8, limiter = 0, device = 0;
+    int waves = 4;             // occupancy target of the step kernel (FP32 fast: 3 or 4)
     hipStream_t stream = nullptr;
     dg::EnsInstance *d_inst = nullptr;
     dg::EnsItem *d_items = nullptr;
     double *partials = nullptr;
     int *d_count = nullptr;    // device: active instances
+    int *d_next = nullptr;     // device: work counter of the step launch
     int *h_count = nullptr;    // pinned mirror
     int nitems = 0, grid = 0;
     int items_cap = 0, slots_cap = 0;  // allocated entries of d_items / partials
@@ -99,10 +103,10 @@ int arm(dg_ensemble *e, long long max_steps, double t_end)
 
 int launch_one(dg_ensemble *e, int parity)
 {
-    if (dg::launch_ens_step(e->prec, e->fast, e->tile_y, e->limiter, e->d_inst, e->d_items, e->nitems, e->grid,
-                            parity, e->partials, e->stream))
+    if (dg::launch_ens_step(e->prec, e->fast, e->tile_y, e->limiter, e->waves, e->d_inst, e->d_items, e->nitems,
+                            e->grid, parity, e->partials, e->d_next, e->stream))
         return efail(e, DG_ERR_DEVICE, std::string("ensemble step launch: ") + hipGetErrorString(hipGetLastError()));
-    if (dg::launch_ens_finalize(e->prec, e->d_inst, (int)e->inst.size(), e->partials, 1, e->stream))
+    if (dg::launch_ens_finalize(e->prec, e->d_inst, (int)e->inst.size(), e->partials, 1, e->d_next, e->stream))
         return efail(e, DG_ERR_DEVICE, "ensemble finalize launch");
     return DG_OK;
 }
@@ -206,7 +210,7 @@ int choose_grid(dg_ensemble *e, int grid)
 {
     if (grid == 0) {
         int cus = 0;
-        const int per_cu = dg::ens_blocks_per_cu(e->prec, e->fast, e->tile_y, e->limiter);
+        const int per_cu = dg::ens_blocks_per_cu(e->prec, e->fast, e->tile_y, e->limiter, e->waves);
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device) != hipSuccess || per_cu <= 0)
             return efail(e, DG_ERR_DEVICE, "occupancy query");
         grid = per_cu * cus;
@@ -269,6 +273,8 @@ int dg_ensemble_create(const dg_config *cfgs, int count, int grid, dg_ensemble *
     e->limiter = f->cfg.limiter;
 
     if (hipMalloc((void **)&e->d_count, sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&e->d_next, sizeof(int)) != hipSuccess ||
+        hipMemset(e->d_next, 0, sizeof(int)) != hipSuccess ||
         hipHostMalloc((void **)&e->h_count, sizeof(int), hipHostMallocDefault) != hipSuccess) {
         dg_ensemble_destroy(e);
         return DG_ERR_ALLOC;
@@ -293,6 +299,7 @@ void dg_ensemble_destroy(dg_ensemble *e)
     if (e->d_items) hipFree(e->d_items);
     if (e->partials) hipFree(e->partials);
     if (e->d_count) hipFree(e->d_count);
+    if (e->d_next) hipFree(e->d_next);
     if (e->h_count) hipHostFree(e->h_count);
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
@@ -389,11 +396,11 @@ int dg_ensemble_time_steps(dg_ensemble *e, int64_t nsteps, double *ms_step_kerne
     for (auto &x : ev) EHIP(e, hipEventCreate(&x));
     for (int64_t s = 0; s < nsteps && rc == DG_OK; ++s) {
         if (hipEventRecord(ev[3 * s], e->stream) != hipSuccess) rc = DG_ERR_DEVICE;
-        if (dg::launch_ens_step(e->prec, e->fast, e->tile_y, e->limiter, e->d_inst, e->d_items, e->nitems, e->grid,
-                                e->parity, e->partials, e->stream))
+        if (dg::launch_ens_step(e->prec, e->fast, e->tile_y, e->limiter, e->waves, e->d_inst, e->d_items, e->nitems,
+                                e->grid, e->parity, e->partials, e->d_next, e->stream))
             rc = efail(e, DG_ERR_DEVICE, "ensemble step launch");
         if (hipEventRecord(ev[3 * s + 1], e->stream) != hipSuccess) rc = DG_ERR_DEVICE;
-        if (dg::launch_ens_finalize(e->prec, e->d_inst, (int)e->inst.size(), e->partials, 1, e->stream))
+        if (dg::launch_ens_finalize(e->prec, e->d_inst, (int)e->inst.size(), e->partials, 1, e->d_next, e->stream))
             rc = efail(e, DG_ERR_DEVICE, "ensemble finalize launch");
         if (hipEventRecord(ev[3 * s + 2], e->stream) != hipSuccess) rc = DG_ERR_DEVICE;
         after_steps(e, 1);
@@ -414,9 +421,11 @@ int dg_ensemble_time_steps(dg_ensemble *e, int64_t nsteps, double *ms_step_kerne
     return DG_OK;
 }
 
-int dg_ensemble_retile(dg_ensemble *e, int tile_y, const int32_t *chunk_x, int grid)
+int dg_ensemble_retile(dg_ensemble *e, int tile_y, const int32_t *chunk_x, int grid, int waves)
 {
-    if (!e || tile_y < 0 || grid < 0) return efail(e, DG_ERR_INVALID, "bad argument");
+    if (!e || tile_y < 0 || grid < 0 || (waves != 0 && waves != 3 && waves != 4))
+        return efail(e, DG_ERR_INVALID, "bad argument");
+    if (waves) e->waves = waves;
     EHIP(e, hipSetDevice(e->device));
     EHIP(e, hipStreamSynchronize(e->stream));
     const int ty = tile_y ? tile_y : e->tile_y;

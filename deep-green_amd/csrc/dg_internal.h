@@ -69,9 +69,11 @@ int launch_finalize(int prec, const StepArgs &a, Scalars *sc, const double *part
 int launch_prim_to_cons(int prec, const StepArgs &a, void *const u[5], hipStream_t s);
 int launch_cons_to_prim(int prec, const StepArgs &a, const void *const u[5], void *const w[5],
                         hipStream_t s);
-int launch_ens_step(int prec, int fast, int tile_y, int limiter, const EnsInstance *inst, const EnsItem *items,
-                    int nitems, int grid, int parity, double *partials, hipStream_t s);
-int launch_ens_finalize(int prec, const EnsInstance *inst, int ninst, const double *partials, int mode,
+// `next`: the work counter (0 before a step launch; the finalize resets it).
+// waves: occupancy target of the FP32 fast variants (3 or 4 waves per SIMD).
+int launch_ens_step(int prec, int fast, int tile_y, int limiter, int waves, const EnsInstance *inst,
+                    const EnsItem *items, int nitems, int grid, int parity, double *partials, int *next, hipStream_t s);
+int launch_ens_finalize(int prec, const EnsInstance *inst, int ninst, const double *partials, int mode, int *next,
                         hipStream_t s);
 // Learned flux correction (FP32 periodic grids): flags cells whose corrected
 // state is not positive, applies the masked correction in place and writes
@@ -85,7 +87,7 @@ int launch_residual(long long n3, const void *const ut[5], const void *const up[
                     void *const r[5], hipStream_t s);
 int launch_set_dt(Scalars *sc, double dt, hipStream_t s);
 // Resident workgroups of the ensemble step kernel per CU (-1 on error).
-int ens_blocks_per_cu(int prec, int fast, int tile_y, int limiter);
+int ens_blocks_per_cu(int prec, int fast, int tile_y, int limiter, int waves);
 int step_lds_bytes(int prec, int tile_y);
 int reduce_blocks(const StepArgs &a);
 

@@ -777,13 +777,14 @@ int dg_cnn_train(dg_cnn *m, int steps, int batch, int L, float lr, uint64_t seed
     return DG_OK;
 }
 
-int dg_learned_step(dg_ctx *c, dg_cnn *m, double *dt_used)
+int dg_learned_step(dg_ctx *c, dg_cnn *m, double dt, double *dt_used)
 {
     if (!c || !m) return DG_ERR_INVALID;
     MHIP(m, hipSetDevice(m->device));
     int rc = prepare_infer(m, c);
     if (rc) return rc;
     hipLaunchKernelGGL(arm_run_kernel, dim3(1), dim3(1), 0, m->stream, c->sc, (double)INFINITY, (long long)LLONG_MAX);
+    if (dt > 0) MCHK(m, dg::launch_set_dt(c->sc, dt, m->stream), "set dt");
     dg::Scalars s;
     MHIP(m, hipMemcpyAsync(&s, c->sc, sizeof(s), hipMemcpyDeviceToHost, m->stream));
     if ((rc = learned_launch(m, c))) return rc;

@@ -795,17 +795,25 @@ step_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> Vout, con
                                       bx, by, bz);
 }
 
-// Persistent ensemble step: workgroups stride over a work list of
-// (instance, tile) items (sorted by cost, largest first); every instance has
-// its own grid, state buffers, scalars and partial slots.
-template <typename T, int TY, bool FAST, bool BULK, int LIM>
-__global__ void __launch_bounds__(TY * TZ) __attribute__((amdgpu_waves_per_eu(waves_per_eu<T, FAST>())))
+// Persistent ensemble step: workgroups take (instance, tile) items from a
+// work list sorted by cost, largest first, through an atomic counter (dynamic
+// load balance); every instance has its own grid, state buffers, scalars and
+// partial slots.
+template <typename T, int TY, bool FAST, bool BULK, int LIM, int W>
+__global__ void __launch_bounds__(TY * TZ) __attribute__((amdgpu_waves_per_eu(W)))
 ens_step_kernel(const EnsInstance *__restrict__ inst, const EnsItem *__restrict__ items, int nitems, int parity,
-                double *__restrict__ partials)
+                double *__restrict__ partials, int *__restrict__ next)
 {
     __shared__ StepLDS<T, TY> L;
     __shared__ double red[(TY * TZ / 64) * 8];
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    __shared__ int s_item;
+    for (;;) {
+        // next item of the (heaviest-first) work list; the counter is reset
+        // by the dt kernel that follows every step launch
+        if (threadIdx.x == 0) s_item = atomicAdd(next, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= nitems) break;
         const EnsItem w = items[it];
         const EnsInstance &I = inst[w.inst];
         const StepArgs A = I.args;
@@ -926,9 +934,11 @@ finalize_kernel(const StepArgs A, Scalars *__restrict__ sc, const double *__rest
 // One workgroup per ensemble instance: its partial slots -> its scalars.
 template <typename T, bool BULK>
 __global__ void __launch_bounds__(256)
-ens_finalize_kernel(const EnsInstance *__restrict__ inst, const double *__restrict__ partials, int mode)
+ens_finalize_kernel(const EnsInstance *__restrict__ inst, const double *__restrict__ partials, int mode,
+                    int *__restrict__ next)
 {
     __shared__ double red[4 * 8];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *next = 0;   // work counter of the next step launch
     const EnsInstance &I = inst[blockIdx.x];
     finalize_body<T, BULK, 256>(I.args.ds, I.sc, partials + (size_t)I.part_base * PARTIAL_STRIDE, I.nparts, I.cfl,
                                 I.nan_fallback, mode, red);
@@ -1137,23 +1147,37 @@ int launch_step(int prec, int fast, int tile_y, const StepArgs &a, const void *c
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-template <typename T, int TY, bool FAST, bool BULK>
-static void launch_ens_t(const EnsInstance *inst, const EnsItem *items, int nitems, int grid, int parity,
-                         int limiter, double *partials, hipStream_t s)
+template <typename T, int TY, bool FAST, bool BULK, int W>
+static void launch_ens_tw(const EnsInstance *inst, const EnsItem *items, int nitems, int grid, int parity,
+                          int limiter, double *partials, int *next, hipStream_t s)
 {
     const dim3 g(grid), b(TY * TZ);
     if (limiter == LIM_MINMOD)
-        hipLaunchKernelGGL((ens_step_kernel<T, TY, FAST, BULK, LIM_MINMOD>), g, b, 0, s, inst, items, nitems, parity, partials);
+        hipLaunchKernelGGL((ens_step_kernel<T, TY, FAST, BULK, LIM_MINMOD, W>), g, b, 0, s, inst, items, nitems, parity, partials, next);
     else if (limiter == LIM_VANLEER)
-        hipLaunchKernelGGL((ens_step_kernel<T, TY, FAST, BULK, LIM_VANLEER>), g, b, 0, s, inst, items, nitems, parity, partials);
+        hipLaunchKernelGGL((ens_step_kernel<T, TY, FAST, BULK, LIM_VANLEER, W>), g, b, 0, s, inst, items, nitems, parity, partials, next);
     else
-        hipLaunchKernelGGL((ens_step_kernel<T, TY, FAST, BULK, LIM_NONE>), g, b, 0, s, inst, items, nitems, parity, partials);
+        hipLaunchKernelGGL((ens_step_kernel<T, TY, FAST, BULK, LIM_NONE, W>), g, b, 0, s, inst, items, nitems, parity, partials, next);
 }
 
-int launch_ens_step(int prec, int fast, int tile_y, int limiter, const EnsInstance *inst, const EnsItem *items,
-                    int nitems, int grid, int parity, double *partials, hipStream_t s)
+// waves: occupancy target of the FP32 fast kernels (3: up to 168 VGPRs, 4:
+// 128); the other variants are compiled without a target.
+template <typename T, int TY, bool FAST, bool BULK>
+static void launch_ens_t(const EnsInstance *inst, const EnsItem *items, int nitems, int grid, int parity,
+                         int limiter, double *partials, int *next, int waves, hipStream_t s)
 {
-#define DG_ENS(T, TYV, F, B) launch_ens_t<T, TYV, F, B>(inst, items, nitems, grid, parity, limiter, partials, s)
+    if constexpr (FAST && sizeof(T) == 4) {
+        if (waves == 3) launch_ens_tw<T, TY, FAST, BULK, 3>(inst, items, nitems, grid, parity, limiter, partials, next, s);
+        else launch_ens_tw<T, TY, FAST, BULK, 4>(inst, items, nitems, grid, parity, limiter, partials, next, s);
+    } else {
+        launch_ens_tw<T, TY, FAST, BULK, 1>(inst, items, nitems, grid, parity, limiter, partials, next, s);
+    }
+}
+
+int launch_ens_step(int prec, int fast, int tile_y, int limiter, int waves, const EnsInstance *inst,
+                    const EnsItem *items, int nitems, int grid, int parity, double *partials, int *next, hipStream_t s)
+{
+#define DG_ENS(T, TYV, F, B) launch_ens_t<T, TYV, F, B>(inst, items, nitems, grid, parity, limiter, partials, next, waves, s)
     if (prec == 0) {
         if (tile_y == 4) DG_ENS(double, 4, false, false);
         else if (tile_y == 8) DG_ENS(double, 8, false, false);
@@ -1175,47 +1199,57 @@ int launch_ens_step(int prec, int fast, int tile_y, int limiter, const EnsInstan
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-template <typename T, int TY, bool FAST, bool BULK>
-static int ens_occ_t(int limiter)
+template <typename T, int TY, bool FAST, bool BULK, int W>
+static int ens_occ_tw(int limiter)
 {
     int n = 0;
     hipError_t e;
     if (limiter == LIM_MINMOD)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ens_step_kernel<T, TY, FAST, BULK, LIM_MINMOD>, TY * TZ, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ens_step_kernel<T, TY, FAST, BULK, LIM_MINMOD, W>, TY * TZ, 0);
     else if (limiter == LIM_VANLEER)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ens_step_kernel<T, TY, FAST, BULK, LIM_VANLEER>, TY * TZ, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ens_step_kernel<T, TY, FAST, BULK, LIM_VANLEER, W>, TY * TZ, 0);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ens_step_kernel<T, TY, FAST, BULK, LIM_NONE>, TY * TZ, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, ens_step_kernel<T, TY, FAST, BULK, LIM_NONE, W>, TY * TZ, 0);
     return e == hipSuccess ? n : -1;
 }
 
-int ens_blocks_per_cu(int prec, int fast, int tile_y, int limiter)
+template <typename T, int TY, bool FAST, bool BULK>
+static int ens_occ_t(int limiter, int waves)
+{
+    if constexpr (FAST && sizeof(T) == 4) {
+        return waves == 3 ? ens_occ_tw<T, TY, FAST, BULK, 3>(limiter) : ens_occ_tw<T, TY, FAST, BULK, 4>(limiter);
+    } else {
+        return ens_occ_tw<T, TY, FAST, BULK, 1>(limiter);
+    }
+}
+
+int ens_blocks_per_cu(int prec, int fast, int tile_y, int limiter, int waves)
 {
     if (prec == 0) {
-        if (tile_y == 4) return ens_occ_t<double, 4, false, false>(limiter);
-        if (tile_y == 8) return ens_occ_t<double, 8, false, false>(limiter);
+        if (tile_y == 4) return ens_occ_t<double, 4, false, false>(limiter, waves);
+        if (tile_y == 8) return ens_occ_t<double, 8, false, false>(limiter, waves);
         return -1;
     }
     const bool bulk = prec == 2;
 #define DG_OCC(TYV)                                                                                    \
     if (tile_y == TYV) {                                                                              \
-        if (fast) return bulk ? ens_occ_t<float, TYV, true, true>(limiter) : ens_occ_t<float, TYV, true, false>(limiter); \
-        return bulk ? ens_occ_t<float, TYV, false, true>(limiter) : ens_occ_t<float, TYV, false, false>(limiter); \
+        if (fast) return bulk ? ens_occ_t<float, TYV, true, true>(limiter, waves) : ens_occ_t<float, TYV, true, false>(limiter, waves); \
+        return bulk ? ens_occ_t<float, TYV, false, true>(limiter, waves) : ens_occ_t<float, TYV, false, false>(limiter, waves); \
     }
     DG_OCC(4) DG_OCC(8) DG_OCC(16)
 #undef DG_OCC
     return -1;
 }
 
-int launch_ens_finalize(int prec, const EnsInstance *inst, int ninst, const double *partials, int mode,
+int launch_ens_finalize(int prec, const EnsInstance *inst, int ninst, const double *partials, int mode, int *next,
                         hipStream_t s)
 {
     if (prec == 0)
-        hipLaunchKernelGGL((ens_finalize_kernel<double, false>), dim3(ninst), dim3(256), 0, s, inst, partials, mode);
+        hipLaunchKernelGGL((ens_finalize_kernel<double, false>), dim3(ninst), dim3(256), 0, s, inst, partials, mode, next);
     else if (prec == 1)
-        hipLaunchKernelGGL((ens_finalize_kernel<float, false>), dim3(ninst), dim3(256), 0, s, inst, partials, mode);
+        hipLaunchKernelGGL((ens_finalize_kernel<float, false>), dim3(ninst), dim3(256), 0, s, inst, partials, mode, next);
     else
-        hipLaunchKernelGGL((ens_finalize_kernel<float, true>), dim3(ninst), dim3(256), 0, s, inst, partials, mode);
+        hipLaunchKernelGGL((ens_finalize_kernel<float, true>), dim3(ninst), dim3(256), 0, s, inst, partials, mode, next);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

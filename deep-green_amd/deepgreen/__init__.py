@@ -110,7 +110,7 @@ def load_library(path=None):
     L.dg_ensemble_clock.argtypes = [vp, i32, P(d), P(i64), P(d)]
     L.dg_ensemble_conservation_sums.argtypes = [vp, i32, P(d)]
     L.dg_ensemble_time_steps.argtypes = [vp, i64, P(d), P(d)]
-    L.dg_ensemble_retile.argtypes = [vp, i32, P(ctypes.c_int32), i32]
+    L.dg_ensemble_retile.argtypes = [vp, i32, P(ctypes.c_int32), i32, i32]
     L.dg_ensemble_info.argtypes = [vp, P(i64), P(i32), P(i32), P(i32)]
     L.dg_device_info.argtypes = [i32, ctypes.c_char_p, i32, P(i32)]
     fp = P(ctypes.c_float)
@@ -135,7 +135,7 @@ def load_library(path=None):
     L.dg_cnn_sample_count.argtypes = [vp]
     L.dg_cnn_fit_normalization.argtypes = [vp]
     L.dg_cnn_train.argtypes = [vp, i32, i32, i32, ctypes.c_float, ctypes.c_uint64, fp]
-    L.dg_learned_step.argtypes = [vp, vp, P(d)]
+    L.dg_learned_step.argtypes = [vp, vp, d, P(d)]
     L.dg_learned_run.argtypes = [vp, vp, i64, d, P(d), P(i64)]
     L.dg_learned_flagged.argtypes = [vp, P(i64)]
     L.dg_learned_correction.argtypes = [vp, vp, fp]
@@ -417,14 +417,16 @@ class Ensemble:
                     "dg_ensemble_time_steps")
         return a.value, b.value
 
-    def retile(self, tile_y=0, chunk_x=None, grid=0):
+    def retile(self, tile_y=0, chunk_x=None, grid=0, waves=0):
         """New tiling (results unchanged): tile_y for all (0 keeps it),
         chunk_x per instance (None keeps them, 0 automatic), grid persistent
-        workgroups (0: device maximum)."""
+        workgroups (0: device maximum), waves per SIMD of the FP32 fast
+        kernel (3 or 4; 0 keeps it)."""
         arr = None
         if chunk_x is not None:
             arr = (ctypes.c_int32 * len(self.sims))(*[int(c) for c in chunk_x])
-        self._check(self._L.dg_ensemble_retile(self._h, int(tile_y), arr, int(grid)), "dg_ensemble_retile")
+        self._check(self._L.dg_ensemble_retile(self._h, int(tile_y), arr, int(grid), int(waves)),
+                    "dg_ensemble_retile")
 
     def info(self):
         cells, items, grid, ty = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -6,15 +6,16 @@ Knobs of one persistent ensemble launch:
   chunk_x        x-planes a workgroup marches, per size class (long chunks
                  re-read fewer halo planes, short ones give more, smaller
                  work items to balance),
-  blocks_per_cu  resident workgroups per CU of the persistent grid (how many
-                 waves per SIMD the launch asks for).
+  waves          occupancy target the FP32 fast step kernel is compiled for
+                 (3 or 4 waves per SIMD: 168 or 128 VGPRs per lane),
+  blocks_per_cu  resident workgroups per CU of the persistent grid.
 
 The tuner builds one probe ensemble with the class mix of the real one
 (scaled down to at most PROBE_CELLS cells, at least one member per class,
 random planar Riemann problems), then re-tiles it in place and times the
-persistent step kernel with HIP events: for every tile_y, coordinate descent
-over the classes' chunk_x (largest class first); the best (tile_y, chunks);
-then the grid.  Results go to a JSON cache keyed by device architecture,
+persistent step kernel with HIP events: for every (tile_y, waves),
+coordinate descent over the classes' chunk_x (largest class first); the best
+(tile_y, waves, chunks); then the grid.  Results go to a JSON cache keyed by device architecture,
 arithmetic (precision/math/limiter) and the set of size classes.  The choice
 never changes results: each instance is bitwise the same under every tiling.
 """
@@ -33,6 +34,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE_PATH = os.environ.get("DG_AUTOTUNE_CACHE", os.path.join(_HERE, "autotune_cache.json"))
 
 TILE_Y = (4, 8, 16)
+WAVES = (4, 3)
 BLOCKS_PER_CU = (1, 2, 0)           # 0: as many as fit
 PROBE_CELLS = 32 * 1024 * 1024
 
@@ -120,39 +122,42 @@ def tune(sizes, counts=None, precision="fp32", math="fast", limiter="minmod", de
         for i, w in enumerate(states):
             ens.set_state(i, w)
 
-        def measure(ty, chunk, bpc):
-            ens.retile(ty, [chunk[n] for n in member_n], grid_for(bpc, device))
+        def measure(ty, wv, chunk, bpc):
+            ens.retile(ty, [chunk[n] for n in member_n], grid_for(bpc, device), wv)
             ens.time_steps(2)
             ms, _ = ens.time_steps(steps)
-            trials.append({"tile_y": ty, "chunk_x": {str(n): chunk[n] for n in sizes}, "blocks_per_cu": bpc,
-                           "ms": round(ms, 5)})
+            trials.append({"tile_y": ty, "waves": wv, "chunk_x": {str(n): chunk[n] for n in sizes},
+                           "blocks_per_cu": bpc, "ms": round(ms, 5)})
             if log:
-                log(f"[autotune] tile_y={ty} chunk_x={chunk} blocks/CU={bpc or 'max'}: {ms:.4f} ms")
+                log(f"[autotune] tile_y={ty} waves={wv} chunk_x={chunk} blocks/CU={bpc or 'max'}: {ms:.4f} ms")
             return ms
 
+        waves = WAVES if (precision != "fp64" and math == "fast") else (4,)
         best = None
         for ty in TILE_Y:
             if precision == "fp64" and ty > 8:
                 continue
-            chunk = {n: 8 if n >= 8 else n for n in sizes}
-            ms_cur = measure(ty, chunk, 0)
-            for n in sorted(set(sizes), reverse=True):
-                for lx in chunk_candidates(n):
-                    if lx == chunk[n]:
-                        continue
-                    trial = dict(chunk)
-                    trial[n] = lx
-                    ms = measure(ty, trial, 0)
-                    if ms < ms_cur:
-                        chunk, ms_cur = trial, ms
-            if best is None or ms_cur < best[2]:
-                best = (ty, chunk, ms_cur, 0)
-        ty, chunk, _, _ = best
+            for wv in waves:
+                chunk = {n: min(32, n) for n in sizes}
+                ms_cur = measure(ty, wv, chunk, 0)
+                for n in sorted(set(sizes), reverse=True):
+                    for lx in chunk_candidates(n):
+                        if lx == chunk[n]:
+                            continue
+                        trial = dict(chunk)
+                        trial[n] = lx
+                        ms = measure(ty, wv, trial, 0)
+                        if ms < ms_cur:
+                            chunk, ms_cur = trial, ms
+                if best is None or ms_cur < best[2]:
+                    best = (ty, chunk, ms_cur, 0, wv)
+        ty, chunk, _, _, wv = best
         for bpc in BLOCKS_PER_CU:
-            ms = measure(ty, chunk, bpc)
+            ms = measure(ty, wv, chunk, bpc)
             if ms < best[2]:
-                best = (ty, chunk, ms, bpc)
-    entry = {"tile_y": best[0], "chunk_x": {str(n): best[1][n] for n in sizes}, "blocks_per_cu": best[3],
+                best = (ty, chunk, ms, bpc, wv)
+    entry = {"tile_y": best[0], "waves": best[4], "chunk_x": {str(n): best[1][n] for n in sizes},
+             "blocks_per_cu": best[3],
              "ms_probe": round(best[2], 5), "compute_units": cus,
              "probe": {"members": {str(n): k for n, k in zip(sizes, per)},
                        "cells": int(sum(k * n ** 3 for n, k in zip(sizes, per))), "steps_timed": steps},
@@ -162,11 +167,8 @@ def tune(sizes, counts=None, precision="fp32", math="fast", limiter="minmod", de
     return entry
 
 
-def apply(sims, choice, device=0):
-    """(Simulations with the chosen tile_y / chunk_x, grid argument)."""
-    out = []
-    for s in sims:
-        s.tile_y = int(choice["tile_y"])
-        s.chunk_x = int(choice["chunk_x"].get(str(s.interior[0]), 0))
-        out.append(s)
-    return out, grid_for(choice["blocks_per_cu"], device)
+def apply(ens, choice, device=0):
+    """Re-tile an Ensemble (its members cubic, keyed by edge length) to an
+    autotuner choice."""
+    chunks = [int(choice["chunk_x"].get(str(s.interior[0]), 0)) for s in ens.sims]
+    ens.retile(int(choice["tile_y"]), chunks, grid_for(choice["blocks_per_cu"], device), int(choice.get("waves", 4)))

@@ -147,10 +147,12 @@ class FluxCNN:
         return losses
 
     # learned solver -----------------------------------------------------------
-    def step(self, solver):
-        dt = ctypes.c_double()
-        self._check(self._L.dg_learned_step(solver.handle, self._h, ctypes.byref(dt)), "dg_learned_step")
-        return dt.value
+    def step(self, solver, dt=0.0):
+        """One learned step (CFL dt, or the given dt when dt > 0); returns the dt used."""
+        used = ctypes.c_double()
+        self._check(self._L.dg_learned_step(solver.handle, self._h, float(dt), ctypes.byref(used)),
+                    "dg_learned_step")
+        return float(dt) if dt > 0 else used.value
 
     def run(self, solver, max_steps, t_end=math.inf):
         t, n = ctypes.c_double(), ctypes.c_int64()
@@ -178,3 +180,116 @@ class FluxCNN:
         G = np.empty(n + (16,), np.float32)
         self._check(self._L.dg_learned_correction(self._h, solver.handle, _fp(G)), "dg_learned_correction")
         return G
+
+
+# ------------------------------------------------------------ workflow --
+def _sim(n, ds):
+    from . import PERIODIC, Simulation
+    return Simulation(n, n, n, ds, [PERIODIC] * 6, gamma=1.4, cfl=0.4, g=0.0, precision="fp32", math="fast",
+                      limiter="minmod")
+
+
+def clock(solver):
+    return solver.run(0)
+
+
+def run_to(solver, T, model=None, guard=4):
+    """Advance a periodic FP32 Solver (plain, or learned with `model`) to t = T
+    exactly: captured-graph runs until a few steps before T, then single
+    steps, the last one with dt = T - t.  Returns the number of steps."""
+    big = 1 << 40
+    t, n0 = clock(solver)
+    dt = solver.next_dt()
+    if t + guard * dt < T:
+        if model is None:
+            solver.run(big, T - guard * dt)
+        else:
+            model.run(solver, big, T - guard * dt)
+    while True:
+        t, n = clock(solver)
+        dt = solver.next_dt()
+        if t + dt >= T:
+            last = T - t
+            if last > 0:
+                if model is None:
+                    solver.step_dt(last)
+                else:
+                    model.step(solver, dt=last)
+            return clock(solver)[1] - n0
+        if model is None:
+            solver.compute_step()
+        else:
+            model.step(solver)
+
+
+def make_training_set(model, seeds, n_fine=256, t_end=0.03, every=8, log=None):
+    """Samples from plain runs of learned_ic(seed) on n_fine^3 against the 2x
+    coarser grid: one sample every `every` fine steps up to t_end."""
+    from . import Solver
+    from .setups import learned_ic
+    for seed in seeds:
+        w, ds = learned_ic(n_fine, seed)
+        fine = Solver(_sim(n_fine, ds))
+        coarse = Solver(_sim(n_fine // 2, 2 * ds))
+        fine.set_primitives(w)
+        del w
+        t = 0.0
+        while t < t_end:
+            model.add_sample(fine, coarse)
+            fine.run(max(every - 2, 0))
+            t = clock(fine)[0]
+        if log:
+            log(f"[learned] seed {seed}: {model.samples} samples, t = {t:.4f}")
+        fine.close()
+        coarse.close()
+
+
+def density(solver):
+    U = solver.get_state()
+    V = solver.sim.ds ** 3
+    return np.asarray(U[0][2:-2, 2:-2, 2:-2], np.float64) / V
+
+
+def evaluate(model, seed, n_truth=512, T=0.03, log=None):
+    """L1 density errors at t = T on the coarsest grid (n_truth / 4) of the
+    plain n_truth/2 and n_truth/4 solvers and of the learned n_truth/4
+    solver, against the plain n_truth solution restricted to that grid (all
+    start from restrictions of the n_truth initial condition), and their
+    wall times (the state resident on the device, graph-captured runs)."""
+    import time
+
+    from . import Solver
+    from .setups import learned_ic
+    n2, n4 = n_truth // 2, n_truth // 4
+    w, ds = learned_ic(n_truth, seed)
+    truth = Solver(_sim(n_truth, ds))
+    truth.set_primitives(w)
+    del w
+    s2, p4, l4 = Solver(_sim(n2, 2 * ds)), Solver(_sim(n4, 4 * ds)), Solver(_sim(n4, 4 * ds))
+    s2.restrict_from(truth)
+    p4.restrict_from(s2)
+    l4.restrict_from(s2)
+    out = {}
+    for name, s, mdl in (("plain_truth", truth, None), ("plain_half", s2, None), ("plain_quarter", p4, None),
+                         ("learned_quarter", l4, model)):
+        s.next_dt()
+        t0 = time.perf_counter()
+        steps = run_to(s, T, mdl)
+        s.next_dt()
+        out[name] = {"steps": int(steps), "wall_s": time.perf_counter() - t0, "n": s.sim.interior[0]}
+        if log:
+            log(f"[learned] {name} {s.sim.interior[0]}^3: {steps} steps in {out[name]['wall_s']:.3f} s")
+    # restrict everything to the coarsest grid
+    r2, r4 = Solver(_sim(n2, 2 * ds)), Solver(_sim(n4, 4 * ds))
+    r2.restrict_from(truth)
+    r4.restrict_from(r2)
+    ref4 = density(r4)
+    q4 = Solver(_sim(n4, 4 * ds))
+    q4.restrict_from(s2)
+    errs = {"plain_half": float(np.abs(density(q4) - ref4).mean()),
+            "plain_quarter": float(np.abs(density(p4) - ref4).mean()),
+            "learned_quarter": float(np.abs(density(l4) - ref4).mean())}
+    mass = (l4.conservation_sums()[0], p4.conservation_sums()[0])
+    for s in (truth, s2, p4, l4, r2, r4, q4):
+        s.close()
+    return {"runs": out, "l1_rho": errs, "mass_learned_vs_plain": mass, "T": T, "seed": seed}

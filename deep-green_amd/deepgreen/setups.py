@@ -115,3 +115,51 @@ def ensemble_workload(count=256, seed=0, precision="fp32", math="fast", limiter=
             yield i, planar_riemann(n, np.random.default_rng(rng.bit_generator.seed_seq), dtype)[0]
 
     return sims, states
+
+
+def learned_ic_params(seed):
+    """Random initial condition of the learned-correction workload: density 1
+    plus 8 Fourier modes (integer wave vectors with components in [-4, 4],
+    amplitudes in [-0.2, 0.2], random phases), pressure 1 plus 1-3 Gaussian
+    blasts of peak 5-20 (width 0.05, periodic distance), at rest."""
+    rng = np.random.default_rng(seed)
+    modes = []
+    while len(modes) < 8:
+        k = rng.integers(-4, 5, size=3)
+        if not k.any():
+            continue
+        modes.append((k.astype(np.float64), rng.uniform(-0.2, 0.2), rng.uniform(0.0, 2 * np.pi)))
+    blasts = [(rng.uniform(0.0, 1.0, size=3), rng.uniform(5.0, 20.0)) for _ in range(int(rng.integers(1, 4)))]
+    return modes, blasts, 0.05
+
+
+def learned_ic(n, seed, dtype=np.float32, rho_min=0.2):
+    """Primitive fields (ghosted, (n+4)^3) of learned_ic_params(seed) at the
+    cell centres of an n^3 periodic unit cube.  Where the 8 modes would take
+    the density below rho_min, the mode sum is scaled down uniformly so that
+    its minimum is rho_min.  Built plane by plane (bounded host memory)."""
+    modes, blasts, sigma = learned_ic_params(seed)
+    c, ds = centres(n)
+    shape = (n + 4,) * 3
+    tw = 2 * np.pi
+    # mode sum, plane by plane in x
+    S = np.empty(shape, np.float64)
+    for i in range(n + 4):
+        acc = np.zeros((n + 4, n + 4))
+        for k, a, ph in modes:
+            acc += a * np.sin(tw * (k[0] * c[i] + k[1] * c[:, None] + k[2] * c[None, :]) + ph)
+        S[i] = acc
+    smin = S.min()
+    scale = 1.0 if 1.0 + smin >= rho_min else (1.0 - rho_min) / (-smin)
+    rho = (1.0 + scale * S).astype(dtype)
+    del S
+    p = np.ones(shape, np.float64)
+    for i in range(n + 4):
+        for ctr, peak in blasts:
+            dx = np.abs(c[i] - ctr[0]); dx = min(dx, 1.0 - dx)
+            dy = np.abs(c - ctr[1]); dy = np.minimum(dy, 1.0 - dy)
+            dz = np.abs(c - ctr[2]); dz = np.minimum(dz, 1.0 - dz)
+            r2 = dx * dx + dy[:, None] ** 2 + dz[None, :] ** 2
+            p[i] += (peak - 1.0) * np.exp(-r2 / (2 * sigma * sigma))
+    z = np.zeros(shape, dtype)
+    return [rho, z, z.copy(), z.copy(), p.astype(dtype)], ds

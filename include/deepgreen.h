@@ -180,8 +180,10 @@ int dg_ensemble_time_steps(dg_ensemble *ens, int64_t nsteps, double *ms_step_ker
 
 /* New tiling without touching the state: tile_y for all instances (0 = keep),
  * chunk_x[i] per instance (chunk_x NULL = keep; 0 = automatic), grid
- * persistent workgroups (0 = as many as fit).  Results do not change. */
-int dg_ensemble_retile(dg_ensemble *ens, int tile_y, const int32_t *chunk_x, int grid);
+ * persistent workgroups (0 = as many as fit), waves = occupancy target of
+ * the FP32 fast step kernel (3 or 4 waves per SIMD; 0 = keep; default 4).
+ * Results do not change. */
+int dg_ensemble_retile(dg_ensemble *ens, int tile_y, const int32_t *chunk_x, int grid, int waves);
 
 /* Interior cells over all instances, work items per step, persistent
  * workgroups, rows per workgroup. */
@@ -248,8 +250,9 @@ int dg_cnn_fit_normalization(dg_cnn *m);
  * samples (batch crops of L^3 cells); losses[steps] (may be NULL). */
 int dg_cnn_train(dg_cnn *m, int steps, int batch, int L, float lr, uint64_t seed, float *losses);
 
-/* dg_step / dg_run with the learned correction (ctx: FP32, periodic). */
-int dg_learned_step(dg_ctx *ctx, dg_cnn *m, double *dt_used);
+/* dg_step / dg_run with the learned correction (ctx: FP32, periodic);
+ * dg_learned_step takes the CFL dt when dt <= 0, else the given dt. */
+int dg_learned_step(dg_ctx *ctx, dg_cnn *m, double dt, double *dt_used);
 int dg_learned_run(dg_ctx *ctx, dg_cnn *m, int64_t max_steps, double t_end, double *t_out, int64_t *steps_out);
 /* A learned step with the correction G [n0*n1*n2][16] given by the host
  * (normalized units, as dg_learned_correction returns) instead of the

@@ -207,5 +207,11 @@ inline emu_f32x4 __builtin_amdgcn_mfma_f32_16x16x32_bf16(emu_bf16x8 a, emu_bf16x
     emu::sync();
     return d;
 }
+inline int atomicAdd(int *p, int v)
+{
+    const int old = *p;   // fibers of one host thread: no concurrent access
+    *p = old + v;
+    return old;
+}
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
