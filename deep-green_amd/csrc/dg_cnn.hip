@@ -90,8 +90,10 @@ template <int CIN, int COUT> struct ConvCfg {
 // EPI_OUT: Y = acc + b (fp32, COUT channels);  EPI_BWD: X = bf16(acc * gelu'(Zprev)).
 enum { EPI_TRAIN = 0, EPI_INFER = 1, EPI_OUT = 2, EPI_BWD = 3 };
 
+constexpr int CONV_WAVES = 8;   // waves per workgroup (they share the weights in LDS)
+
 template <int CIN, int COUT, int EPI, int PAD>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * CONV_WAVES)
 conv_kernel(const ConvGeom g, const bf16 *__restrict__ X, const bf16x8 *__restrict__ Wp,
             const float *__restrict__ bias, float *__restrict__ Zout, bf16 *__restrict__ Xout,
             float *__restrict__ Yout, const float *__restrict__ Zprev)
@@ -99,7 +101,7 @@ conv_kernel(const ConvGeom g, const bf16 *__restrict__ X, const bf16x8 *__restri
     typedef ConvCfg<CIN, COUT> C;
     typedef Mfma<COUT> M;
     __shared__ bf16x8 wl[C::KS * 64];
-    for (int i = threadIdx.x; i < C::KS * 64; i += 256) wl[i] = Wp[i];
+    for (int i = threadIdx.x; i < C::KS * 64; i += 64 * CONV_WAVES) wl[i] = Wp[i];
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -107,8 +109,9 @@ conv_kernel(const ConvGeom g, const bf16 *__restrict__ X, const bf16x8 *__restri
     const int i1 = g.in[1], i2 = g.in[2];
     const int o0 = g.out[0], o1 = g.out[1], o2 = g.out[2];
     const long long plane = (long long)i1 * i2;
-    const long long stride = (long long)gridDim.x * 4 * C::WAVE_CELLS;
-    for (long long base = ((long long)blockIdx.x * 4 + wave) * C::WAVE_CELLS; base < g.ncells; base += stride) {
+    const long long stride = (long long)gridDim.x * CONV_WAVES * C::WAVE_CELLS;
+    for (long long base = ((long long)blockIdx.x * CONV_WAVES + wave) * C::WAVE_CELLS; base < g.ncells;
+         base += stride) {
         long long inb[C::NT];
         int ox[C::NT], oy[C::NT], oz[C::NT];
         bool ok[C::NT];
@@ -125,14 +128,8 @@ conv_kernel(const ConvGeom g, const bf16 *__restrict__ X, const bf16x8 *__restri
             const long long b = r2 / o0;
             inb[t] = ((b * g.in[0] + ox[t] - PAD) * i1 + oy[t] - PAD) * i2 + oz[t] - PAD;
         }
-        typename M::acc_t acc[C::NT];
-#pragma unroll
-        for (int t = 0; t < C::NT; ++t)
-#pragma unroll
-            for (int i = 0; i < M::NACC; ++i) acc[t][i] = 0.0f;
-
-        for (int s = 0; s < C::KS; ++s) {
-            const bf16x8 a = wl[s * 64 + lane];
+        // B operands of k-step s (this lane's 8 channels of its cells)
+        auto load_b = [&](int s, bf16x8 *bv) {
             const int k = s * C::KSTEP + part * 8;
             const int tap = k / CIN, c0 = k % CIN;
             const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;
@@ -144,10 +141,25 @@ conv_kernel(const ConvGeom g, const bf16 *__restrict__ X, const bf16x8 *__restri
                     const int ix = ox[t] + dx - PAD, iy = oy[t] + dy - PAD, iz = oz[t] + dz - PAD;
                     in = in && ix >= 0 && ix < g.in[0] && iy >= 0 && iy < i1 && iz >= 0 && iz < i2;
                 }
-                bf16x8 bv = zero8();
-                if (in) bv = *reinterpret_cast<const bf16x8 *>(X + (inb[t] + off) * CIN + c0);
-                acc[t] = M::op(a, bv, acc[t]);
+                bv[t] = in ? *reinterpret_cast<const bf16x8 *>(X + (inb[t] + off) * CIN + c0) : zero8();
             }
+        };
+        typename M::acc_t acc[C::NT];
+#pragma unroll
+        for (int t = 0; t < C::NT; ++t)
+#pragma unroll
+            for (int i = 0; i < M::NACC; ++i) acc[t][i] = 0.0f;
+
+        // software pipeline: the loads of k-step s+1 are in flight during the MFMAs of s
+        bf16x8 bcur[C::NT], bnext[C::NT];
+        load_b(0, bcur);
+        for (int s = 0; s < C::KS; ++s) {
+            if (s + 1 < C::KS) load_b(s + 1, bnext);
+            const bf16x8 a = wl[s * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < C::NT; ++t) acc[t] = M::op(a, bcur[t], acc[t]);
+#pragma unroll
+            for (int t = 0; t < C::NT; ++t) bcur[t] = bnext[t];
         }
 
         // epilogue: lane holds output channels M::row(i, lane) of cell base + t*TILE + col
@@ -514,12 +526,13 @@ int conv(int layer, int mode, int B, const int in[3], const void *X, const void 
 {
     const int pad = mode == CONV_BWD ? 2 : 0;
     const ConvGeom g = geom(B, in, pad);
-    const int blocks = grid_for(g.ncells, 256, 8192);
+    const int blocks = grid_for(g.ncells, 64 * CONV_WAVES, 8192);
     const bf16 *x = static_cast<const bf16 *>(X);
     const bf16x8 *w = static_cast<const bf16x8 *>(Wp);
     bf16 *xo = static_cast<bf16 *>(Xout);
-#define DG_CONV(CI, CO, E, P) \
-    hipLaunchKernelGGL((conv_kernel<CI, CO, E, P>), dim3(blocks), dim3(256), 0, s, g, x, w, bias, Z, xo, Y, Zprev)
+#define DG_CONV(CI, CO, E, P)                                                                                 \
+    hipLaunchKernelGGL((conv_kernel<CI, CO, E, P>), dim3(blocks), dim3(64 * CONV_WAVES), 0, s, g, x, w, bias, Z, xo, \
+                       Y, Zprev)
     if (layer == 1) {
         if (mode == CONV_TRAIN) DG_CONV(8, 32, EPI_TRAIN, 0);
         else if (mode == CONV_INFER) DG_CONV(8, 32, EPI_INFER, 0);
