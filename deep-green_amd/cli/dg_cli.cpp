@@ -1,0 +1,368 @@
+// dg_cli -- the reference's command-line host (src/rs/src/main.rs) on the
+// MI355X step library: the same demo set-ups (src/rs/src/init.rs, with their
+// ghost-inclusive coordinates), the same loop and output cadence
+// (main.rs:38-68: dump after the step when wtime > tout or on the first
+// step, then tnow += dt in f32), the state written per dump.
+//
+//   dg_cli --demo KELVIN_HELMHOLTZ --tout 0.1 --total-time 1.0 --max-steps 100
+//          [--precision fp32|fp64|fp32_bulk] [--math exact|fast]
+//          [--limiter none|minmod|vanleer] [--out DIR] [--no-dump] [--quiet]
+//
+// Dumps: DIR/state.NNNNN/ (main.rs:58-60) with one .npy file per dataset of
+// the reference's HDF5 layout (io.rs:5-196): primitives.{rho,vx,vy,vz,pressure}
+// and conserved.{mass,momentum_x,momentum_y,momentum_z,energy}, float64, full
+// ghosted shape (nx+4, ny+4, nz+4).  The reference's gradients and fluxes
+// groups are not written: the fused step never materializes them.
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../include/deepgreen.h"
+
+namespace {
+
+// reference constants (globals.rs)
+const float kG = -0.1f, kDensity = 1.293f, kPressure = 101325.0f, kRs = 287.0f;
+const float kPi = 3.14159265358979323846f;
+
+struct Grid {
+    int nx, ny, nz;  // ghosted sizes (Simulation::nx etc., grid.rs:64-66)
+    float ds;
+    std::vector<float> f[5];  // rho, vx, vy, vz, p
+    Grid(int ix, int iy, int iz, float d) : nx(ix + 4), ny(iy + 4), nz(iz + 4), ds(d)
+    {
+        for (auto &v : f) v.assign((size_t)nx * ny * nz, 0.0f);
+    }
+    float &at(int q, int i, int j, int k) { return f[q][((size_t)i * ny + j) * nz + k]; }
+    void fill(int q, float v) { for (auto &x : f[q]) x = v; }
+};
+
+struct Demo {
+    int n[3];
+    float ds;
+    int bc[6];
+    bool gravity;
+    std::function<void(Grid &)> init;
+};
+
+enum { P = DG_PERIODIC, O = DG_OUTFLOW, W = DG_WALL };
+
+// init_khi (init.rs:401-444)
+void init_khi(Grid &g)
+{
+    const float kappa = 0.1f, s = 0.05f / std::sqrt(2.0f);
+    g.fill(4, 2.5f);
+    g.fill(2, 0.0f);
+    const float H = g.ds * g.nz;
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                const float xs = i * g.ds, zs = k * g.ds;
+                const float in = std::fabs(zs - H / 2.0f) < H / 4.0f ? 1.0f : 0.0f;
+                g.at(0, i, j, k) = 1.0f + in;
+                g.at(1, i, j, k) = -0.5f + in;
+                g.at(3, i, j, k) = kappa * std::sin(4.0f * kPi * xs) *
+                                   (std::exp(-std::pow(zs - H / 4.0f, 2.0f) / (2.0f * s * s)) +
+                                    std::exp(-std::pow(zs - 3.0f * H / 4.0f, 2.0f) / (2.0f * s * s)));
+            }
+}
+
+// the two lowest z planes of p from plane 2 plus hydrostatic increments
+void hydrostatic_floor(Grid &g)
+{
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j) {
+            const float p2 = g.at(4, i, j, 2);
+            g.at(4, i, j, 1) = p2 + std::fabs(kG) * g.ds;
+            g.at(4, i, j, 0) = p2 + 2.0f * std::fabs(kG) * g.ds;
+        }
+}
+
+// init_rt (init.rs:446-486)
+void init_rt(Grid &g)
+{
+    const float xmin = -0.25f, zmin = -0.75f, p0 = 2.5f;
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                const float xs = xmin + g.ds * i, zs = zmin + g.ds * k;
+                const float rho = zs > 0.0f ? 2.0f : 1.0f;
+                g.at(0, i, j, k) = rho;
+                g.at(4, i, j, k) = p0 - 0.1f * rho * 1.4f;
+                g.at(3, i, j, k) =
+                    -0.01f * (1.0f + std::cos(4.0f * kPi * xs)) * (1.0f + std::cos(3.0f * kPi * zs)) / 4.0f;
+            }
+    hydrostatic_floor(g);
+}
+
+// init_rb (init.rs:323-366)
+void init_rb(Grid &g)
+{
+    const float rho0 = 1.0f, rsp = 287.052874f, t0 = 373.0f, tf = t0 - 20.0f;
+    const float p0 = rho0 * rsp * t0, pf = rho0 * rsp * tf;
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j) {
+            float noise = (std::cos(8.0f * kPi * j / g.ny) + std::sin(6.0f * kPi * i / g.nx)) * 0.05f;
+            for (int k = 0; k < g.nz; ++k) {
+                const float zs = k * g.ds;
+                if (!(zs < 0.2f)) noise = 0.0f;
+                g.at(0, i, j, k) = rho0 + noise;
+                g.at(4, i, j, k) = p0 + ((pf - p0) / (g.nz * g.ds) * zs);
+            }
+        }
+}
+
+// init_explosion_2d (init.rs:696-724)
+void init_explosion_2d(Grid &g)
+{
+    g.fill(0, 1.0f);
+    g.fill(4, 2.5f);
+    const float sigma = 0.0001f;
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                const float xs = ((i - g.nx / 2.0f) * g.ds) / g.nx;
+                const float ys = ((j - g.ny / 2.0f) * g.ds) / g.nx;
+                const float zs = ((k - g.nz / 2.0f) * g.ds) / g.nx;
+                g.at(4, i, j, k) += 0.1f * std::exp(-(xs * xs + ys * ys + zs * zs) / (2.0f * sigma * sigma));
+            }
+}
+
+// init_relaxation_3d (init.rs:726-732)
+void init_relaxation(Grid &g)
+{
+    g.fill(0, kDensity);
+    g.fill(4, kPressure);
+}
+
+// init_shock_tube (init.rs:734-757)
+void init_shock_tube(Grid &g)
+{
+    const float length = g.nx * g.ds;
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                const bool left = i * g.ds < length / 2.0f;
+                g.at(4, i, j, k) = left ? 1.0f : 0.1f;
+                g.at(0, i, j, k) = left ? 1.0f : 0.125f;
+            }
+}
+
+// hydrostatic column of the rising-bubble demos (init.rs:759-790)
+void column(Grid &g)
+{
+    const float p0 = 101000.0f, t0 = 293.0f;
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                const float zs = (g.nz - 2.0f * 2) * g.ds + (0.0f * g.ds) - (k - 2.0f) * g.ds;
+                g.at(4, i, j, k) = p0 + std::fabs(kG) * zs;
+                g.at(0, i, j, k) = g.at(4, i, j, k) / (kRs * t0);
+            }
+    hydrostatic_floor(g);
+}
+
+void bubble(Grid &g, const float c[3], float r_out, float r_in)
+{
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                const float mag = std::sqrt(std::pow(i - c[0], 2.0f) + std::pow(j - c[1], 2.0f) + std::pow(k - c[2], 2.0f));
+                if (mag < r_out) g.at(0, i, j, k) = 1.0f;
+                if (mag < r_in) g.at(0, i, j, k) = 0.8f;
+            }
+}
+
+// init_trb_demo (init.rs:759-812)
+void init_trb(Grid &g)
+{
+    column(g);
+    const float c[3] = {75.0f, 1.0f, 128.0f};
+    bubble(g, c, 25.0f, 20.0f);
+}
+
+// init_trbs_demo (init.rs:814-896)
+void init_trbs(Grid &g)
+{
+    column(g);
+    const float c1[3] = {32.0f + 8.0f, 1.0f, 32.0f}, c2[3] = {88.0f - 8.0f, 1.0f, 32.0f}, c3[3] = {64.0f, 1.0f, 100.0f};
+    bubble(g, c1, 15.0f, 10.0f);
+    bubble(g, c2, 15.0f, 10.0f);
+    bubble(g, c3, 1.5f * 15.0f, 1.5f * 10.0f);
+}
+
+// set_up_*_demo (init.rs:8-321) and the dispatch of main.rs:79-108
+bool find_demo(const std::string &name, Demo &d)
+{
+    if (name == "KELVIN_HELMHOLTZ") d = {{256, 2, 128}, 1.0f / 256, {P, P, P, P, P, P}, false, init_khi};
+    else if (name == "RAYLEIGH_TAYLOR") d = {{100, 2, 300}, 2.0f * 0.0025f, {P, P, P, P, W, W}, true, init_rt};
+    else if (name == "RAYLEIGH_BERNARD") d = {{64, 64, 64}, 1.0f / 64, {W, W, W, W, W, W}, true, init_rb};
+    else if (name == "EXPLOSION") d = {{128, 2, 128}, 1.0f / 128, {W, W, P, P, W, W}, false, init_explosion_2d};
+    else if (name == "RELAXATION") d = {{30, 30, 30}, 0.1f, {W, W, W, W, W, O}, true, init_relaxation};
+    else if (name == "SHOCK_TUBE") d = {{5000, 2, 2}, 0.0002f, {W, W, P, P, P, P}, false, init_shock_tube};
+    else if (name == "THERMAL_RISING_BUBBLE") d = {{150, 2, 512}, 1.0f, {W, W, P, P, W, O}, true, init_trb};
+    else if (name == "THERMAL_RISING_BUBBLES") d = {{128, 2, 500}, 2.0f, {W, W, P, P, W, O}, true, init_trbs};
+    else return false;
+    return true;
+}
+
+bool write_npy(const std::string &path, const std::vector<double> &v, int nx, int ny, int nz)
+{
+    FILE *fp = std::fopen(path.c_str(), "wb");
+    if (!fp) return false;
+    char hdr[256];
+    int len = std::snprintf(hdr, sizeof(hdr), "{'descr': '<f8', 'fortran_order': False, 'shape': (%d, %d, %d), }",
+                            nx, ny, nz);
+    int total = 10 + len + 1;
+    const int pad = (64 - total % 64) % 64;
+    std::string h(hdr, len);
+    h.append(pad, ' ');
+    h.push_back('\n');
+    const unsigned short hl = (unsigned short)h.size();
+    std::fwrite("\x93NUMPY\x01\x00", 1, 8, fp);
+    std::fwrite(&hl, 2, 1, fp);
+    std::fwrite(h.data(), 1, h.size(), fp);
+    const bool ok = std::fwrite(v.data(), sizeof(double), v.size(), fp) == v.size();
+    return std::fclose(fp) == 0 && ok;
+}
+
+// save_to_disk (io.rs:5-196): primitives and conserved, f64, ghosted
+int dump(dg_ctx *ctx, const Grid &shape, int precision, const std::string &dir)
+{
+    static const char *names[2][5] = {{"rho", "vx", "vy", "vz", "pressure"},
+                                      {"mass", "momentum_x", "momentum_y", "momentum_z", "energy"}};
+    static const char *groups[2] = {"primitives", "conserved"};
+    mkdir(dir.c_str(), 0755);
+    const size_t n = (size_t)shape.nx * shape.ny * shape.nz;
+    const size_t el = precision == DG_FP64 ? 8 : 4;
+    std::vector<std::vector<char>> buf(5, std::vector<char>(n * el));
+    for (int kind = 0; kind < 2; ++kind) {
+        void *f[5];
+        for (int q = 0; q < 5; ++q) f[q] = buf[q].data();
+        int rc = dg_get_state(ctx, f, kind == 0 ? DG_PRIMITIVE : DG_CONSERVED);
+        if (rc) return rc;
+        for (int q = 0; q < 5; ++q) {
+            std::vector<double> v(n);
+            for (size_t i = 0; i < n; ++i)
+                v[i] = el == 8 ? reinterpret_cast<const double *>(buf[q].data())[i]
+                               : (double)reinterpret_cast<const float *>(buf[q].data())[i];
+            const std::string path = dir + "/" + groups[kind] + "." + names[kind][q] + ".npy";
+            if (!write_npy(path, v, shape.nx, shape.ny, shape.nz)) return DG_ERR_INVALID;
+        }
+    }
+    return DG_OK;
+}
+
+void usage()
+{
+    std::fprintf(stderr,
+                 "usage: dg_cli --demo NAME --tout T --total-time T --max-steps N\n"
+                 "              [--precision fp32|fp64|fp32_bulk] [--math exact|fast]\n"
+                 "              [--limiter none|minmod|vanleer] [--out DIR] [--no-dump] [--quiet]\n"
+                 "demos: KELVIN_HELMHOLTZ THERMAL_RISING_BUBBLE THERMAL_RISING_BUBBLES EXPLOSION RELAXATION\n"
+                 "       SHOCK_TUBE RAYLEIGH_TAYLOR RAYLEIGH_BERNARD\n");
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    std::string demo, out = ".", precision = "fp32", math = "exact", limiter = "none";
+    float tout = -1, total_time = -1;
+    long long max_steps = -1;
+    bool do_dump = true, quiet = false;
+    for (int a = 1; a < argc; ++a) {
+        const std::string k = argv[a];
+        auto val = [&]() -> std::string {
+            if (a + 1 >= argc) { usage(); std::exit(2); }
+            return argv[++a];
+        };
+        if (k == "--demo" || k == "-d") demo = val();
+        else if (k == "--tout" || k == "-t") tout = std::strtof(val().c_str(), nullptr);
+        else if (k == "--total-time") total_time = std::strtof(val().c_str(), nullptr);
+        else if (k == "--max-steps" || k == "-m") max_steps = std::strtoll(val().c_str(), nullptr, 10);
+        else if (k == "--precision") precision = val();
+        else if (k == "--math") math = val();
+        else if (k == "--limiter") limiter = val();
+        else if (k == "--out") out = val();
+        else if (k == "--no-dump") do_dump = false;
+        else if (k == "--quiet") quiet = true;
+        else { usage(); return 2; }
+    }
+    if (demo.empty() || tout < 0 || total_time < 0 || max_steps < 0) { usage(); return 2; }
+    Demo d;
+    if (!find_demo(demo, d)) {
+        std::fprintf(stderr, "Demo not found! Available demos are:\n");
+        usage();
+        return 0;   // the reference exits 0 here too (main.rs:103-108)
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    dg_config cfg;
+    dg_config_default(&cfg);   // GAMMA, CFL, G of globals.rs
+    for (int i = 0; i < 3; ++i) cfg.n[i] = d.n[i];
+    for (int i = 0; i < 6; ++i) cfg.bc[i] = d.bc[i];
+    cfg.ds = d.ds;
+    cfg.gravity = d.gravity;
+    cfg.precision = precision == "fp64" ? DG_FP64 : (precision == "fp32_bulk" ? DG_FP32_BULK : DG_FP32);
+    cfg.math = math == "fast" ? DG_MATH_FAST : DG_MATH_EXACT;
+    cfg.limiter = limiter == "minmod" ? DG_LIMITER_MINMOD : (limiter == "vanleer" ? DG_LIMITER_VANLEER : DG_LIMITER_NONE);
+    dg_ctx *ctx = nullptr;
+    int rc = dg_create(&cfg, &ctx);
+    if (rc) { std::fprintf(stderr, "dg_create failed (%d)\n", rc); return 1; }
+    Grid g(d.n[0], d.n[1], d.n[2], d.ds);
+    d.init(g);
+    std::printf("Running simulation %s: %dx%dx%d cells, ds %g.\n", demo.c_str(), d.n[0], d.n[1], d.n[2], (double)d.ds);
+    // state upload (calc_conserved of main.rs:50 happens in the library)
+    std::vector<std::vector<char>> host(5);
+    const size_t n = (size_t)g.nx * g.ny * g.nz, el = cfg.precision == DG_FP64 ? 8 : 4;
+    const void *f[5];
+    for (int q = 0; q < 5; ++q) {
+        host[q].resize(n * el);
+        for (size_t i = 0; i < n; ++i) {
+            if (el == 8) reinterpret_cast<double *>(host[q].data())[i] = g.f[q][i];
+            else reinterpret_cast<float *>(host[q].data())[i] = g.f[q][i];
+        }
+        f[q] = host[q].data();
+    }
+    if ((rc = dg_set_state(ctx, f, DG_PRIMITIVE))) {
+        std::fprintf(stderr, "dg_set_state: %s\n", dg_last_error(ctx));
+        return 1;
+    }
+    // compute() of main.rs:38-68
+    float tnow = 0.0f, dt = 0.5f, wtime = 0.0f;
+    long long step = 0, wstep = 0;
+    while (!(tnow > total_time || step >= max_steps)) {
+        if (!quiet) std::printf("Computing step %lld and time %g with dt %g. \n", step, (double)tnow, (double)dt);
+        double dtd = 0;
+        if ((rc = dg_step(ctx, &dtd))) {
+            std::fprintf(stderr, "dg_step: %s\n", dg_last_error(ctx));
+            return 1;
+        }
+        dt = (float)dtd;
+        if (wtime > tout || wstep == 0) {
+            char name[32];
+            std::snprintf(name, sizeof(name), "state.%05lld", wstep);
+            if (do_dump && (rc = dump(ctx, g, cfg.precision, out + "/" + name))) {
+                std::fprintf(stderr, "dump failed (%d)\n", rc);
+                return 1;
+            }
+            wtime = 0.0f;
+            wstep += 1;
+        }
+        tnow += dt;
+        wtime += dt;
+        step += 1;
+    }
+    dg_destroy(ctx);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("Simulation done in %.3fs! (%lld steps, t = %g, %lld dumps)\n", secs, step, (double)tnow, wstep);
+    return 0;
+}

@@ -854,6 +854,47 @@ int dg_learned_step_given(dg_cnn *m, dg_ctx *c, const float *G, double *dt_used)
     return DG_OK;
 }
 
+int dg_learned_time(dg_cnn *m, dg_ctx *c, int nrep, double ms[5])
+{
+    if (!m || !c || nrep <= 0 || !ms) return DG_ERR_INVALID;
+    MHIP(m, hipSetDevice(m->device));
+    int rc = prepare_infer(m, c);
+    if (rc) return rc;
+    const int n[3] = {c->args.n[0], c->args.n[1], c->args.n[2]};
+    const int in0[3] = {n[0] + 6, n[1] + 6, n[2] + 6}, in1[3] = {n[0] + 4, n[1] + 4, n[2] + 4},
+              in2[3] = {n[0] + 2, n[1] + 2, n[2] + 2};
+    hipEvent_t ev[6];
+    for (auto &e : ev) MHIP(m, hipEventCreate(&e));
+    double acc[5] = {0, 0, 0, 0, 0};
+    dg::cnn::PrepArgs a = prep_args(m, 1, n, 0, 3, c->cfg.ds, c->cfg.gamma);
+    for (int d = 0; d < 3; ++d) a.dims[d] = in0[d];
+    for (int r = 0; r < nrep && rc == DG_OK; ++r) {
+        MHIP(m, hipEventRecord(ev[0], m->stream));
+        MCHK(m, dg::cnn::prep_input(a, m->d_ptrs + 5 * c->cur, m->d_box0, m->ws[WS_X0].p, m->stream), "prep");
+        MHIP(m, hipEventRecord(ev[1], m->stream));
+        MCHK(m, dg::cnn::conv(1, dg::cnn::CONV_INFER, 1, in0, m->ws[WS_X0].p, m->Wp[0], m->bias_pad[0], nullptr,
+                              m->ws[WS_IX1].p, nullptr, nullptr, m->stream), "conv1");
+        MHIP(m, hipEventRecord(ev[2], m->stream));
+        MCHK(m, dg::cnn::conv(2, dg::cnn::CONV_INFER, 1, in1, m->ws[WS_IX1].p, m->Wp[1], m->bias_pad[1], nullptr,
+                              m->ws[WS_IX2].p, nullptr, nullptr, m->stream), "conv2");
+        MHIP(m, hipEventRecord(ev[3], m->stream));
+        MCHK(m, dg::cnn::conv(3, dg::cnn::CONV_OUT, 1, in2, m->ws[WS_IX2].p, m->Wp[2], m->bias_pad[2], nullptr,
+                              nullptr, (float *)m->ws[WS_IG].p, nullptr, m->stream), "conv3");
+        MHIP(m, hipEventRecord(ev[4], m->stream));
+        MHIP(m, hipEventSynchronize(ev[4]));
+        float t;
+        for (int k = 0; k < 4; ++k) {
+            hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+            acc[k] += t;
+        }
+        hipEventElapsedTime(&t, ev[0], ev[4]);
+        acc[4] += t;
+    }
+    for (auto &e : ev) hipEventDestroy(e);
+    for (int k = 0; k < 5; ++k) ms[k] = acc[k] / nrep;
+    return DG_OK;
+}
+
 int dg_learned_flagged(dg_cnn *m, int64_t *count)
 {
     if (!m || !count || !m->flags || !m->graph_ctx) return DG_ERR_INVALID;

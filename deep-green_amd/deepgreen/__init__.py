@@ -43,7 +43,7 @@ EXPORTS = ("dg_config_default", "dg_create", "dg_destroy", "dg_last_error", "dg_
            "dg_cnn_set_params", "dg_cnn_get_normalization", "dg_cnn_set_normalization", "dg_cnn_forward",
            "dg_cnn_backward", "dg_cnn_train_batch", "dg_cnn_adam_step", "dg_cnn_add_sample", "dg_cnn_sample_count",
            "dg_cnn_fit_normalization", "dg_cnn_train", "dg_learned_step", "dg_learned_run", "dg_learned_flagged",
-           "dg_learned_correction", "dg_learned_step_given")
+           "dg_learned_correction", "dg_learned_step_given", "dg_learned_time")
 
 
 class DgConfig(ctypes.Structure):
@@ -140,6 +140,7 @@ def load_library(path=None):
     L.dg_learned_flagged.argtypes = [vp, P(i64)]
     L.dg_learned_correction.argtypes = [vp, vp, fp]
     L.dg_learned_step_given.argtypes = [vp, vp, fp, P(d)]
+    L.dg_learned_time.argtypes = [vp, vp, i32, P(d)]
     for name in EXPORTS:
         if name not in ("dg_config_default", "dg_destroy", "dg_last_error", "dg_ensemble_destroy",
                         "dg_ensemble_last_error", "dg_cnn_destroy", "dg_cnn_last_error", "dg_cnn_param_count"):

@@ -169,6 +169,13 @@ class FluxCNN:
                     "dg_learned_step_given")
         return dt.value
 
+    def time_forward(self, solver, nrep=10):
+        """Mean ms of [input preparation, layer 1, layer 2, layer 3, total]
+        of the network's forward on the solver's grid (HIP events)."""
+        ms = (ctypes.c_double * 5)()
+        self._check(self._L.dg_learned_time(self._h, solver.handle, int(nrep), ms), "dg_learned_time")
+        return list(ms)
+
     def flagged(self):
         n = ctypes.c_int64()
         self._check(self._L.dg_learned_flagged(self._h, ctypes.byref(n)), "dg_learned_flagged")
@@ -250,7 +257,7 @@ def density(solver):
     return np.asarray(U[0][2:-2, 2:-2, 2:-2], np.float64) / V
 
 
-def evaluate(model, seed, n_truth=512, T=0.03, warmup_steps=0, log=None):
+def evaluate(model, seed, n_truth=512, T=0.03, warmup_steps=0, log=None, ic=None):
     """L1 density errors at t = T on the coarsest grid (n_truth / 4) of the
     plain n_truth/2 and n_truth/4 solvers and of the learned n_truth/4
     solver, against the plain n_truth solution restricted to that grid (all
@@ -264,7 +271,9 @@ def evaluate(model, seed, n_truth=512, T=0.03, warmup_steps=0, log=None):
     from . import Solver
     from .setups import learned_ic
     n2, n4 = n_truth // 2, n_truth // 4
-    w, ds = learned_ic(n_truth, seed)
+    w, ds = ic if ic is not None else learned_ic(n_truth, seed)
+    if log:
+        log(f"[learned] evaluation IC {n_truth}^3 (seed {seed}) ready")
     truth = Solver(_sim(n_truth, ds))
     truth.set_primitives(w)
     del w

@@ -258,6 +258,10 @@ int dg_learned_run(dg_ctx *ctx, dg_cnn *m, int64_t max_steps, double t_end, doub
  * (normalized units, as dg_learned_correction returns) instead of the
  * network's: for tests of the correction, the fallback and conservation. */
 int dg_learned_step_given(dg_cnn *m, dg_ctx *ctx, const float *G, double *dt_used);
+/* Timing harness: the network's forward on the state of ctx, nrep times, HIP
+ * events around each launch; mean ms of [input preparation, layer 1, layer 2,
+ * layer 3, total]. */
+int dg_learned_time(dg_cnn *m, dg_ctx *ctx, int nrep, double ms[5]);
 /* Cells that kept the plain flux in the last learned step. */
 int dg_learned_flagged(dg_cnn *m, int64_t *count);
 /* Network output for the current state of ctx: G [n0*n1*n2][16] (channel
