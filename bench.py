@@ -204,11 +204,23 @@ def bench_learned(args):
     train_s = time.perf_counter() - t0
     log(f"[learned] {m.samples} samples in {gen_s:.1f} s; {args.train_steps} Adam steps in {train_s:.1f} s, "
         f"loss {losses[:5].mean():.4g} -> {losses[-50:].mean():.4g}")
-    res = evaluate(m, seed=1000 + args.seed, n_truth=4 * args.n_learned, T=args.t_end, log=log)
+    res = evaluate(m, seed=1000 + args.seed, n_truth=4 * args.n_learned, T=args.t_end,
+                   warmup_steps=args.learned_warmup, log=log)
     e, r = res["l1_rho"], res["runs"]
     speedup = r["plain_half"]["wall_s"] / r["learned_quarter"]["wall_s"]
     matched = e["learned_quarter"] <= e["plain_half"]
     n = args.n_learned
+    # dominant kernel of the learned step: the 32 -> 32 convolution (MFMA)
+    import deepgreen as dg
+    from deepgreen.learned import _sim
+    from deepgreen.setups import learned_ic
+    w, ds = learned_ic(n, 1000 + args.seed)
+    s = dg.Solver(_sim(n, ds))
+    s.set_primitives(w)
+    ms = m.time_forward(s, nrep=20)
+    s.close()
+    flops = (n + 2) ** 3 * 2 * 27 * 32 * 32
+    tflops = flops / (ms[2] * 1e-3) / 1e12
     return {
         "metric": "error-matched speedup learned vs plain", "unit": "x", "value": speedup, "dtype": "f32",
         "s_per_step": r["learned_quarter"]["wall_s"] / max(r["learned_quarter"]["steps"], 1),
@@ -217,8 +229,12 @@ def bench_learned(args):
                    "samples": m.samples, "train_steps": args.train_steps, "batch": 8, "crop": 32, "lr": 1e-3,
                    "loss_first": float(losses[:5].mean()), "loss_last": float(losses[-50:].mean()),
                    "data_gen_s": round(gen_s, 2), "train_s": round(train_s, 2),
-                   "mass_learned_vs_plain": res["mass_learned_vs_plain"]},
-        "roofline": None,
+                   "mass_learned_vs_plain": res["mass_learned_vs_plain"], "warmup_steps": args.learned_warmup,
+                   "forward_ms": {"prep": round(ms[0], 4), "conv1": round(ms[1], 4), "conv2": round(ms[2], 4),
+                                  "conv3": round(ms[3], 4), "total": round(ms[4], 4)}},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(tflops / 2500.0, 4), "traffic": None, "kernel": "conv_kernel<32, 32, 1, 0>",
+                     "kernel_ms": round(ms[2], 4), "flop_per_launch": flops},
     }
 
 
@@ -233,6 +249,7 @@ def main():
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--sample-every", type=int, default=8)
     ap.add_argument("--t-end", type=float, default=0.03)
+    ap.add_argument("--learned-warmup", type=int, default=0, help="plain steps before the correction starts")
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)

@@ -8,6 +8,7 @@
 // positions through the boundary conditions.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -255,6 +256,8 @@ int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out)
 
     dg_ctx *c = new (std::nothrow) dg_ctx();
     if (!c) return DG_ERR_ALLOC;
+    static std::atomic<uint64_t> next_uid{1};
+    c->uid = next_uid.fetch_add(1);
     c->cfg = *cfg;
     c->prec = cfg->precision;
     c->fast = cfg->math == DG_MATH_FAST;

@@ -4,12 +4,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <stdint.h>
+
 #include <string>
 
 #include "../../include/deepgreen.h"
 #include "dg_internal.h"
 
 struct dg_ctx {
+    uint64_t uid = 0;     // unique per created context (never reused, unlike the address)
     dg_config cfg{};
     int prec = 1, fast = 0, tile_y = 8, chunk_x = 8;
     dg::StepArgs args{};

@@ -79,7 +79,8 @@ struct dg_cnn {
     int *d_box0 = nullptr;
     unsigned char *flags = nullptr;
     size_t flags_n = 0;
-    dg_ctx *graph_ctx = nullptr;
+    uint64_t graph_uid = 0;   // context the tables and captured steps belong to
+    size_t graph_n3 = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
 };
 
@@ -296,13 +297,14 @@ int prepare_infer(dg_cnn *m, dg_ctx *c)
         const int zero[4] = {0, 0, 0, 0};
         MHIP(m, hipMemcpy(m->d_box0, zero, sizeof(zero), hipMemcpyHostToDevice));
     }
-    if (m->graph_ctx != c) {
+    if (m->graph_uid != c->uid) {
         drop_graphs(m);
         const float *p[10];
         for (int b = 0; b < 2; ++b)
             for (int q = 0; q < 5; ++q) p[5 * b + q] = static_cast<const float *>(c->buf[b][q]);
         MHIP(m, hipMemcpy(m->d_ptrs, p, sizeof(p), hipMemcpyHostToDevice));
-        m->graph_ctx = c;
+        m->graph_uid = c->uid;
+        m->graph_n3 = c->n3;
     }
     if (m->flags_n < c->n3) {
         if (m->flags) hipFree(m->flags);
@@ -897,8 +899,8 @@ int dg_learned_time(dg_cnn *m, dg_ctx *c, int nrep, double ms[5])
 
 int dg_learned_flagged(dg_cnn *m, int64_t *count)
 {
-    if (!m || !count || !m->flags || !m->graph_ctx) return DG_ERR_INVALID;
-    std::vector<unsigned char> h(m->graph_ctx->n3);
+    if (!m || !count || !m->flags || !m->graph_uid) return DG_ERR_INVALID;
+    std::vector<unsigned char> h(m->graph_n3);
     MHIP(m, hipMemcpyAsync(h.data(), m->flags, h.size(), hipMemcpyDeviceToHost, m->stream));
     MHIP(m, hipStreamSynchronize(m->stream));
     int64_t n = 0;

@@ -231,3 +231,27 @@ def test_learned_sample_and_training_smoke(emu_lib):
     s1 = coarse.conservation_sums()
     np.testing.assert_allclose(s1[0], s0[0], rtol=1e-5)
     fine.close(), coarse.close()
+
+
+def test_learned_run_after_solver_replaced(emu_lib):
+    """A model used on one grid and then on a new grid of the same size (the
+    first one destroyed, so its memory may be reused at the same addresses)
+    rebuilds its input tables and captured steps: the second grid's run
+    equals that of a fresh model."""
+    from deepgreen.learned import FluxCNN
+    m = model(emu_lib, seed=11)
+    p = m.get_params()
+    a = periodic_solver(emu_lib, 5, 1)
+    m.run(a, 16)
+    a.close()
+    b = periodic_solver(emu_lib, 5, 2)
+    m.run(b, 16)
+    got = interior(b.get_state())
+    b.close()
+    m2 = FluxCNN(seed=12, lib=emu_lib)
+    m2.set_params(p)
+    m2.set_normalization(m.get_normalization())
+    c = periodic_solver(emu_lib, 5, 2)
+    m2.run(c, 16)
+    np.testing.assert_array_equal(got, interior(c.get_state()))
+    c.close()
