@@ -78,6 +78,30 @@ def cpu_baseline(n_sample=128, steps=3):
             "sample": f"KHI {n_sample}^3, {steps} steps, {what}, {wall:.2f} s"}
 
 
+def cpu_baseline_ensemble(members=32, steps=10, seed=0):
+    """CPU path for the ensemble workload: the C oracle (a port of
+    physics.rs with the minmod slopes the ensemble uses; the reference has no
+    limited scheme to time) on the first `members` members of the seeded
+    ensemble, all host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import dg_oracle
+    from deepgreen.setups import ensemble_members, planar_riemann
+    cores = os.cpu_count() or 1
+    cells, wall = 0, 0.0
+    for n, rng in ensemble_members(256, seed)[:members]:
+        w, ds, _ = planar_riemann(n, np.random.default_rng(rng.bit_generator.seed_seq), np.float64)
+        o = dg_oracle.Oracle((n,) * 3, ds, [1] * 6, precision="f32", limiter="minmod", nthreads=cores)
+        o.set_primitives(w)
+        o.calc_conserved()
+        t0 = time.perf_counter()
+        o.steps(steps)
+        wall += time.perf_counter() - t0
+        cells += n ** 3 * steps
+    return {"value": cells / wall / 1e9, "unit": "Gcell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"first {members} ensemble members, {steps} steps each, C oracle (physics.rs port, minmod), "
+                      f"OpenMP, {wall:.2f} s"}
+
+
 def bench_khi(args):
     import deepgreen as dg
     from deepgreen.setups import khi
@@ -286,7 +310,7 @@ def main():
         r = bench_khi(args)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.cpu_sample)
+        cpu = cpu_baseline_ensemble() if args.workload == "ensemble" else cpu_baseline(args.cpu_sample)
     out = {
         "metric": r.get("metric", "Gcell-updates/s + % HBM peak"),
         "value": round(r["value"], 3),

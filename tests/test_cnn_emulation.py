@@ -255,3 +255,17 @@ def test_learned_run_after_solver_replaced(emu_lib):
     m2.run(c, 16)
     np.testing.assert_array_equal(got, interior(c.get_state()))
     c.close()
+
+
+def test_learned_run_equals_single_learned_steps(emu_lib):
+    """dg_learned_run (captured graphs of 16 steps plus single steps) equals
+    the same number of dg_learned_step calls, bit for bit, with a non-zero
+    correction."""
+    m = model(emu_lib, seed=13)
+    a, b = periodic_solver(emu_lib, 5, 3), periodic_solver(emu_lib, 5, 3)
+    m.run(a, 17)
+    for _ in range(17):
+        m.step(b)
+    np.testing.assert_array_equal(interior(a.get_state()), interior(b.get_state()))
+    assert a.run(0) == b.run(0)
+    a.close(), b.close()
