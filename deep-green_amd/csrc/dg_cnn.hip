@@ -38,6 +38,7 @@ namespace cnn {
 
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -162,25 +163,42 @@ conv_kernel(const ConvGeom g, const bf16 *__restrict__ X, const bf16x8 *__restri
             for (int t = 0; t < C::NT; ++t) bcur[t] = bnext[t];
         }
 
-        // epilogue: lane holds output channels M::row(i, lane) of cell base + t*TILE + col
+        // epilogue: lane holds output channels M::row(i, lane) of cell base + t*TILE + col,
+        // in runs of 4 consecutive channels (accumulators 4g .. 4g+3): vector stores
 #pragma unroll
         for (int t = 0; t < C::NT; ++t) {
             if (!ok[t]) continue;
             const long long m = base + t * C::TILE + col;
 #pragma unroll
-            for (int i = 0; i < M::NACC; ++i) {
-                const int ch = M::row(i, lane);
-                float v = acc[t][i];
-                if (EPI == EPI_TRAIN) {
-                    v += bias[ch];
-                    Zout[m * COUT + ch] = v;
-                    Xout[m * COUT + ch] = (bf16)gelu(v);
-                } else if (EPI == EPI_INFER) {
-                    Xout[m * COUT + ch] = (bf16)gelu(v + bias[ch]);
-                } else if (EPI == EPI_OUT) {
-                    Yout[m * COUT + ch] = v + bias[ch];
+            for (int gq = 0; gq < M::NACC / 4; ++gq) {
+                const int ch = M::row(4 * gq, lane);
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[r] = acc[t][4 * gq + r];
+                if (EPI == EPI_BWD) {
+                    const f32x4 zp = *reinterpret_cast<const f32x4 *>(Zprev + m * COUT + ch);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) v[r] *= gelu_grad(zp[r]);
                 } else {
-                    Xout[m * COUT + ch] = (bf16)(v * gelu_grad(Zprev[m * COUT + ch]));
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) v[r] += bias[ch + r];
+                }
+                if (EPI == EPI_TRAIN) {
+                    f32x4 z;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) z[r] = v[r];
+                    *reinterpret_cast<f32x4 *>(Zout + m * COUT + ch) = z;
+                }
+                if (EPI == EPI_OUT) {
+                    f32x4 y;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) y[r] = v[r];
+                    *reinterpret_cast<f32x4 *>(Yout + m * COUT + ch) = y;
+                } else {
+                    bf16x4 x;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) x[r] = (bf16)(EPI == EPI_BWD ? v[r] : gelu(v[r]));
+                    *reinterpret_cast<bf16x4 *>(Xout + m * COUT + ch) = x;
                 }
             }
         }
