@@ -27,6 +27,8 @@ struct PrepArgs {
     float ds, gamma;
     float shift[5], inv_scale[5];   // primitive normalization
     float inv_out[5];               // target normalization
+    float noise;                    // std of Gaussian noise added to the normalized inputs (training)
+    unsigned int seed;              // noise stream
 };
 
 enum { CONV_TRAIN = 0, CONV_INFER = 1, CONV_OUT = 2, CONV_BWD = 3 };

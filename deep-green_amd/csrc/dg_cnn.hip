@@ -49,6 +49,17 @@ __device__ __forceinline__ float gelu_grad(float z)
     return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
+// 32-bit integer hash (xorshift-multiply finalizer)
+__device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
 __device__ __forceinline__ bf16x8 zero8()
 {
     bf16x8 v;
@@ -420,7 +431,17 @@ prep_input_kernel(const PrepArgs a, const float *const *__restrict__ samples, co
         const float q[5] = {rho, vx, vy, vz, p};
         bf16x8 out = zero8();
 #pragma unroll
-        for (int f = 0; f < 5; ++f) out[f] = (bf16)((q[f] - a.shift[f]) * a.inv_scale[f]);
+        for (int f = 0; f < 5; ++f) {
+            float v = (q[f] - a.shift[f]) * a.inv_scale[f];
+            if (a.noise > 0.0f) {
+                // counter-based normal variate (Box-Muller on two hashed uniforms)
+                const uint32_t k = (uint32_t)(e * 5 + f);
+                const uint32_t h1 = mix32(k ^ a.seed), h2 = mix32(k ^ (a.seed * 0x9E3779B9u + 0x7F4A7C15u));
+                const float u1 = ((h1 >> 8) + 1u) * (1.0f / 16777217.0f), u2 = (h2 >> 8) * (1.0f / 16777216.0f);
+                v += a.noise * sqrtf(-2.0f * logf(u1)) * cosf(6.2831853f * u2);
+            }
+            out[f] = (bf16)v;
+        }
         *reinterpret_cast<bf16x8 *>(X + e * 8) = out;
     }
 }

@@ -62,6 +62,7 @@ struct dg_cnn {
     long long adam_t = 0;
     float norm[15];          // shift[5], scale[5], out_scale[5]
     bool norm_set = false;
+    float input_noise = 0.0f;  // training-time input noise (normalized units)
     // samples (device, fp32 SoA interior): U^n and R
     int sn[3] = {0, 0, 0};
     double sds = 0, sgamma = 1.4;
@@ -702,6 +703,13 @@ int dg_cnn_add_sample(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse)
 
 int dg_cnn_sample_count(dg_cnn *m) { return m ? (int)(m->sU.size() / 5) : 0; }
 
+int dg_cnn_set_input_noise(dg_cnn *m, float std)
+{
+    if (!m || !(std >= 0.0f)) return DG_ERR_INVALID;
+    m->input_noise = std;
+    return DG_OK;
+}
+
 int dg_cnn_fit_normalization(dg_cnn *m)
 {
     if (!m) return DG_ERR_INVALID;
@@ -760,6 +768,8 @@ int dg_cnn_train(dg_cnn *m, int steps, int batch, int L, float lr, uint64_t seed
     for (int s = 0; s < steps; ++s) {
         const int *bx = boxes + (size_t)s * batch * 4;
         dg::cnn::PrepArgs a = prep_args(m, batch, m->sn, L + 7, 4, m->sds, m->sgamma);
+        a.noise = m->input_noise;
+        a.seed = (unsigned int)(splitmix(st) & 0xffffffffu);
         MCHK(m, dg::cnn::prep_input(a, m->d_sU, bx, X0, m->stream), "prep input");
         dg::cnn::PrepArgs at = prep_args(m, batch, m->sn, L, 0, m->sds, m->sgamma);
         MCHK(m, dg::cnn::prep_target(at, m->d_sR, bx, T, m->stream), "prep target");

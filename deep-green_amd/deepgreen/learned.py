@@ -140,6 +140,10 @@ class FluxCNN:
     def fit_normalization(self):
         self._check(self._L.dg_cnn_fit_normalization(self._h), "dg_cnn_fit_normalization")
 
+    def set_input_noise(self, std):
+        """Gaussian noise (normalized units) added to the inputs while training."""
+        self._check(self._L.dg_cnn_set_input_noise(self._h, float(std)), "dg_cnn_set_input_noise")
+
     def train(self, steps, batch=8, crop=32, lr=1e-3, seed=0):
         losses = np.empty(int(steps), np.float32)
         self._check(self._L.dg_cnn_train(self._h, int(steps), int(batch), int(crop), float(lr), int(seed),

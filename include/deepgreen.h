@@ -244,6 +244,9 @@ int dg_cnn_adam_step(dg_cnn *m, const float *grads, float lr, float beta1, float
  * FP32 and periodic; the fine state advances by two steps. */
 int dg_cnn_add_sample(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse);
 int dg_cnn_sample_count(dg_cnn *m);
+/* Standard deviation of Gaussian noise added to the normalized network inputs
+ * during dg_cnn_train (0 = none; training-time augmentation). */
+int dg_cnn_set_input_noise(dg_cnn *m, float std);
 /* Normalization from the samples (mean/std of the primitives, RMS of R). */
 int dg_cnn_fit_normalization(dg_cnn *m);
 /* steps of Adam (beta 0.9/0.999, eps 1e-8) on random periodic crops of the
