@@ -32,7 +32,8 @@
 
 namespace {
 
-const int kEnsGraphSteps = 16;  // even: the buffer parity is unchanged by a replay
+const int kEnsGraphSteps = 16;    // even: the buffer parity is unchanged by a replay
+const int kWholeRunSteps = 1024;  // runs up to this long (no stop time) are one captured graph
 
 }  // namespace
 
@@ -52,6 +53,8 @@ struct dg_ensemble {
     int parity = 0;            // state buffer of every instance
     int64_t cells = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // captured steps, by state buffer parity
+    struct RunGraph { int parity; int64_t steps; hipGraphExec_t exec; };
+    std::vector<RunGraph> runs;   // whole runs captured as one graph
     std::string err;
 };
 
@@ -120,24 +123,55 @@ void after_steps(dg_ensemble *e, int64_t nsteps)
     }
 }
 
-int ensure_graph(dg_ensemble *e)
+// nsteps steps from the current parity, captured into one executable graph
+int capture(dg_ensemble *e, int64_t nsteps, hipGraphExec_t *out)
 {
-    if (e->graph[e->parity]) return DG_OK;
+    *out = nullptr;
     hipGraph_t g = nullptr;
     EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     int rc = DG_OK;
-    for (int s = 0; s < kEnsGraphSteps && rc == DG_OK; ++s) rc = launch_one(e, e->parity ^ (s & 1));
+    for (int64_t s = 0; s < nsteps && rc == DG_OK; ++s) rc = launch_one(e, e->parity ^ (int)(s & 1));
     hipError_t he = hipStreamEndCapture(e->stream, &g);
     if (rc) { if (g) hipGraphDestroy(g); return rc; }
     if (he != hipSuccess) return efail(e, DG_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(he));
-    he = hipGraphInstantiate(&e->graph[e->parity], g, nullptr, nullptr, 0);
+    he = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (he != hipSuccess) {
-        e->graph[e->parity] = nullptr;
+        *out = nullptr;
         return efail(e, DG_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(he));
     }
     return DG_OK;
 }
+
+int ensure_graph(dg_ensemble *e)
+{
+    if (e->graph[e->parity]) return DG_OK;
+    return capture(e, kEnsGraphSteps, &e->graph[e->parity]);
+}
+
+// The graph of a whole run of nsteps from the current parity (cached).
+int run_graph(dg_ensemble *e, int64_t nsteps, hipGraphExec_t *out)
+{
+    for (const auto &r : e->runs)
+        if (r.parity == e->parity && r.steps == nsteps) { *out = r.exec; return DG_OK; }
+    int rc = capture(e, nsteps, out);
+    if (rc) return rc;
+    e->runs.push_back({e->parity, nsteps, *out});
+    return DG_OK;
+}
+
+void drop_graphs(dg_ensemble *e)
+{
+    for (int b = 0; b < 2; ++b)
+        if (e->graph[b]) {
+            hipGraphExecDestroy(e->graph[b]);
+            e->graph[b] = nullptr;
+        }
+    for (auto &r : e->runs) hipGraphExecDestroy(r.exec);
+    e->runs.clear();
+}
+
+void drop_graphs(dg_ensemble *e);
 
 // Instance table and work list from the instances' current tiling: every
 // (instance, tile) is one item with its own partial slot; items are ordered
@@ -198,11 +232,7 @@ int build_worklist(dg_ensemble *e)
     EHIP(e, hipMemcpy(e->d_inst, tab.data(), tab.size() * sizeof(dg::EnsInstance), hipMemcpyHostToDevice));
     EHIP(e, hipMemcpy(e->d_items, sorted.data(), sorted.size() * sizeof(dg::EnsItem), hipMemcpyHostToDevice));
     e->nitems = (int)sorted.size();
-    for (int b = 0; b < 2; ++b)
-        if (e->graph[b]) {
-            hipGraphExecDestroy(e->graph[b]);
-            e->graph[b] = nullptr;
-        }
+    drop_graphs(e);
     return DG_OK;
 }
 
@@ -292,8 +322,7 @@ void dg_ensemble_destroy(dg_ensemble *e)
 {
     if (!e) return;
     if (e->stream) hipStreamSynchronize(e->stream);
-    for (int b = 0; b < 2; ++b)
-        if (e->graph[b]) hipGraphExecDestroy(e->graph[b]);
+    drop_graphs(e);
     for (dg_ctx *c : e->inst) dg_destroy(c);
     if (e->d_inst) hipFree(e->d_inst);
     if (e->d_items) hipFree(e->d_items);
@@ -333,6 +362,14 @@ int dg_ensemble_run(dg_ensemble *e, int64_t max_steps, double t_end)
     const bool timed = std::isfinite(t_end);
     const int n = (int)e->inst.size();
     int64_t left = max_steps;
+    if (!timed && max_steps > 1 && max_steps <= kWholeRunSteps) {
+        // the whole run is one graph launch
+        hipGraphExec_t g = nullptr;
+        if ((rc = run_graph(e, max_steps, &g))) return rc;
+        EHIP(e, hipGraphLaunch(g, e->stream));
+        after_steps(e, max_steps);
+        left = 0;
+    }
     while (left > 0) {
         int64_t did;
         if (left >= kEnsGraphSteps) {

@@ -165,7 +165,9 @@ int dg_ensemble_set_state(dg_ensemble *ens, int i, const void *const f[5], int k
 int dg_ensemble_get_state(dg_ensemble *ens, int i, void *const f[5], int kind);
 
 /* Up to max_steps steps of every instance; an instance whose own clock
- * passes t_end (the test of dg_run) stops advancing while the others go on. */
+ * passes t_end (the test of dg_run) stops advancing while the others go on.
+ * Without a stop time (t_end infinite) a run of up to 1024 steps is a single
+ * launch of one captured graph (cached per length). */
 int dg_ensemble_run(dg_ensemble *ens, int64_t max_steps, double t_end);
 
 /* Run clock of instance i (as dg_run reports it) and the dt of its next step. */
