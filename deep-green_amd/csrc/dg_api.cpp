@@ -152,8 +152,10 @@ int dg_create(const dg_config *cfg, dg_ctx **out) { return dg::create_ctx(cfg, n
 namespace dg {
 
 // Workgroup tiling of a context: tile_y rows (4, 8, 16; FP64 4, 8), chunk_x
-// x-planes per workgroup (0: enough chunks for ~2048 workgroups, at least 8
-// planes each).  Drops captured graphs (they bake in the grid).
+// x-planes per workgroup (0: the longest chunks that still give ~512
+// workgroups -- two resident per CU of a 256-CU device, one wave of
+// workgroups -- and at least 8 planes; long chunks re-read fewer halo
+// planes).  Drops captured graphs (they bake in the grid).
 int set_tiling(dg_ctx *c, int tile_y, int chunk_x)
 {
     if (tile_y != 4 && tile_y != 8 && tile_y != 16) return DG_ERR_INVALID;
@@ -165,7 +167,7 @@ int set_tiling(dg_ctx *c, int tile_y, int chunk_x)
     int lx = chunk_x;
     if (lx <= 0) {
         const long long yz = (long long)a.nby * a.nbz;
-        const long long want = (long long)a.n[0] * yz / 2048;
+        const long long want = (long long)a.n[0] * yz / 512;
         lx = (int)(want < 8 ? 8 : want);
     }
     if (lx > a.n[0]) lx = a.n[0];

@@ -72,7 +72,7 @@ typedef struct dg_config {
     int32_t dt_nan_fallback; /* 1: a NaN wave speed gives dt = CFL*1.0 (physics.rs:191-194) */
     int32_t device;        /* HIP device ordinal */
     int32_t tile_y;        /* 0 = automatic; else rows per workgroup (4, 8 or 16) */
-    int32_t chunk_x;       /* 0 = automatic; else planes per workgroup along x */
+    int32_t chunk_x;       /* 0 = automatic (long chunks, ~512 workgroups); else planes per workgroup along x */
 } dg_config;
 
 typedef struct dg_ctx dg_ctx;
