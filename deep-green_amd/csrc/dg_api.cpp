@@ -516,8 +516,9 @@ int dg_time_steps(dg_ctx *c, int64_t nsteps, double *ms_total, double *ms_step_k
     if (!c->have_state) return fail(c, DG_ERR_STATE, "no state set");
     HIPCHK(c, hipSetDevice(c->cfg.device));
     hipLaunchKernelGGL(arm_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, (double)INFINITY, (long long)LLONG_MAX, 0);
-    std::vector<hipEvent_t> ev((size_t)nsteps * 2 + 1);
-    for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+    dg::EventSet es;
+    HIPCHK(c, es.create((size_t)nsteps * 2 + 1));
+    std::vector<hipEvent_t> &ev = es.ev;
     int rc = DG_OK;
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
     for (int64_t s = 0; s < nsteps && rc == DG_OK; ++s) {
@@ -543,7 +544,6 @@ int dg_time_steps(dg_ctx *c, int64_t nsteps, double *ms_total, double *ms_step_k
         tr += ms;
     }
     hipEventElapsedTime(&ms, ev[0], ev[2 * nsteps]);
-    for (auto &e : ev) hipEventDestroy(e);
     if (rc) return rc;
     if (ms_total) *ms_total = ms;
     if (ms_step_kernel) *ms_step_kernel = tk / nsteps;

@@ -429,8 +429,9 @@ int dg_ensemble_time_steps(dg_ensemble *e, int64_t nsteps, double *ms_step_kerne
     if (rc) return rc;
     EHIP(e, hipSetDevice(e->device));
     if ((rc = arm(e, LLONG_MAX, INFINITY))) return rc;
-    std::vector<hipEvent_t> ev((size_t)nsteps * 3);
-    for (auto &x : ev) EHIP(e, hipEventCreate(&x));
+    dg::EventSet es;
+    EHIP(e, es.create((size_t)nsteps * 3));
+    std::vector<hipEvent_t> &ev = es.ev;
     for (int64_t s = 0; s < nsteps && rc == DG_OK; ++s) {
         if (hipEventRecord(ev[3 * s], e->stream) != hipSuccess) rc = DG_ERR_DEVICE;
         if (dg::launch_ens_step(e->prec, e->fast, e->tile_y, e->limiter, e->waves, e->d_inst, e->d_items, e->nitems,
@@ -451,7 +452,6 @@ int dg_ensemble_time_steps(dg_ensemble *e, int64_t nsteps, double *ms_step_kerne
         hipEventElapsedTime(&ms, ev[3 * s + 1], ev[3 * s + 2]);
         tf += ms;
     }
-    for (auto &x : ev) hipEventDestroy(x);
     if (rc) return rc;
     if (ms_step_kernel) *ms_step_kernel = tk / nsteps;
     if (ms_dt_kernel) *ms_dt_kernel = tf / nsteps;

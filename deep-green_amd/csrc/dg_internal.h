@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace dg {
 
 enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_WALL = 2 };
@@ -41,6 +43,25 @@ struct Scalars {
 };
 
 template <typename T> struct Fields { T *f[5]; };
+
+// HIP events of a timing harness, destroyed on every exit path.
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    hipError_t create(size_t n)
+    {
+        ev.assign(n, nullptr);
+        for (auto &e : ev) {
+            const hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) { e = nullptr; return r; }
+        }
+        return hipSuccess;
+    }
+    ~EventSet()
+    {
+        for (auto &e : ev)
+            if (e) hipEventDestroy(e);
+    }
+};
 
 // One grid of a batched ensemble (device-resident table).
 struct EnsInstance {

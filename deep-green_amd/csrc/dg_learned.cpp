@@ -875,8 +875,9 @@ int dg_learned_time(dg_cnn *m, dg_ctx *c, int nrep, double ms[5])
     const int n[3] = {c->args.n[0], c->args.n[1], c->args.n[2]};
     const int in0[3] = {n[0] + 6, n[1] + 6, n[2] + 6}, in1[3] = {n[0] + 4, n[1] + 4, n[2] + 4},
               in2[3] = {n[0] + 2, n[1] + 2, n[2] + 2};
-    hipEvent_t ev[6];
-    for (auto &e : ev) MHIP(m, hipEventCreate(&e));
+    dg::EventSet es;
+    MHIP(m, es.create(6));
+    std::vector<hipEvent_t> &ev = es.ev;
     double acc[5] = {0, 0, 0, 0, 0};
     dg::cnn::PrepArgs a = prep_args(m, 1, n, 0, 3, c->cfg.ds, c->cfg.gamma);
     for (int d = 0; d < 3; ++d) a.dims[d] = in0[d];
@@ -902,7 +903,6 @@ int dg_learned_time(dg_cnn *m, dg_ctx *c, int nrep, double ms[5])
         hipEventElapsedTime(&t, ev[0], ev[4]);
         acc[4] += t;
     }
-    for (auto &e : ev) hipEventDestroy(e);
     for (int k = 0; k < 5; ++k) ms[k] = acc[k] / nrep;
     return DG_OK;
 }
