@@ -132,6 +132,20 @@ int dg_device_count(int *count)
     return DG_OK;
 }
 
+int dg_set_heat(dg_ctx *c, double t_out_kelvin, double v_out, double rs)
+{
+    if (!c || !(t_out_kelvin >= 0) || !(v_out >= 0) || !(rs > 0)) return DG_ERR_INVALID;
+    c->heat_t_out = t_out_kelvin;
+    c->heat_v_out = v_out;
+    c->heat_rs = rs;
+    for (int b = 0; b < 2; ++b)
+        if (c->graph[b]) {   // the captured steps hold the constants
+            hipGraphExecDestroy(c->graph[b]);
+            c->graph[b] = nullptr;
+        }
+    return DG_OK;
+}
+
 int dg_device_info(int device, char *arch, int len, int *compute_units)
 {
     if (!arch || len <= 0) return DG_ERR_INVALID;
@@ -213,7 +227,18 @@ int launch_one_step(dg_ctx *c)
     for (int f = 0; f < 5; ++f) { in[f] = c->buf[c->cur][f]; out[f] = c->buf[1 - c->cur][f]; }
     int rc = dg::launch_step(c->prec, c->fast, c->tile_y, c->args, in, out, c->sc, c->partials, c->stream);
     if (rc) return fail(c, DG_ERR_DEVICE, std::string("step launch: ") + hipGetErrorString(hipGetLastError()));
-    rc = dg::launch_finalize(c->prec, c->args, c->sc, c->partials, c->args.nblocks, c->cfg.cfl,
+    int nparts = c->args.nblocks;
+    if (c->heat_mask) {
+        // heat through conducting walls, then the next dt from the heated state
+        if (dg::launch_heat(c->prec, c->args, in, out, c->sc, c->heat_t_out, c->heat_v_out, c->heat_rs,
+                            c->heat_mask, c->stream))
+            return fail(c, DG_ERR_DEVICE, "heat launch");
+        const void *u[5] = {out[0], out[1], out[2], out[3], out[4]};
+        if (dg::launch_reduce(c->prec, c->fast, c->args, u, c->partials, c->nparts_reduce, c->stream))
+            return fail(c, DG_ERR_DEVICE, "reduce launch");
+        nparts = c->nparts_reduce;
+    }
+    rc = dg::launch_finalize(c->prec, c->args, c->sc, c->partials, nparts, c->cfg.cfl,
                              c->cfg.dt_nan_fallback, 1, c->stream);
     if (rc) return fail(c, DG_ERR_DEVICE, "finalize launch");
     c->cur = 1 - c->cur;
@@ -244,7 +269,7 @@ int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out)
     for (int d = 0; d < 3; ++d)
         if (cfg->n[d] < 2 || cfg->n[d] > (1 << 20)) return DG_ERR_INVALID;
     for (int i = 0; i < 6; ++i)
-        if (cfg->bc[i] < 0 || cfg->bc[i] > 2) return DG_ERR_INVALID;
+        if (cfg->bc[i] < 0 || cfg->bc[i] > 3) return DG_ERR_INVALID;
     for (int d = 0; d < 3; ++d)  // periodic faces pair up (grid.rs:55-63)
         if ((cfg->bc[2 * d] == DG_PERIODIC) != (cfg->bc[2 * d + 1] == DG_PERIODIC)) return DG_ERR_INVALID;
     if (!(cfg->ds > 0) || !(cfg->gamma > 1) || !(cfg->cfl > 0)) return DG_ERR_INVALID;
@@ -270,7 +295,10 @@ int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out)
 
     dg::StepArgs &a = c->args;
     for (int d = 0; d < 3; ++d) a.n[d] = (int)cfg->n[d];
-    for (int i = 0; i < 6; ++i) a.bc[i] = cfg->bc[i];
+    // a conducting wall is a wall for the flow; a lower one carries the heat source
+    for (int i = 0; i < 6; ++i) a.bc[i] = cfg->bc[i] == DG_CONDUCTING_WALL ? DG_WALL : cfg->bc[i];
+    for (int d = 0; d < 3; ++d)
+        if (cfg->bc[2 * d] == DG_CONDUCTING_WALL) c->heat_mask |= 1 << d;
     a.limiter = cfg->limiter;
     a.gravity = cfg->gravity;
     a.ds = cfg->ds;
@@ -528,7 +556,16 @@ int dg_time_steps(dg_ctx *c, int64_t nsteps, double *ms_total, double *ms_step_k
         if (dg::launch_step(c->prec, c->fast, c->tile_y, c->args, in, out, c->sc, c->partials, c->stream))
             rc = fail(c, DG_ERR_DEVICE, "step launch");
         if (hipEventRecord(ev[2 * s + 1], c->stream) != hipSuccess) rc = DG_ERR_DEVICE;
-        if (dg::launch_finalize(c->prec, c->args, c->sc, c->partials, c->args.nblocks, c->cfg.cfl,
+        int nparts = c->args.nblocks;
+        if (c->heat_mask) {   // conducting walls: heat, then the dt of the heated state
+            const void *u2[5] = {out[0], out[1], out[2], out[3], out[4]};
+            if (dg::launch_heat(c->prec, c->args, in, out, c->sc, c->heat_t_out, c->heat_v_out, c->heat_rs,
+                                c->heat_mask, c->stream) ||
+                dg::launch_reduce(c->prec, c->fast, c->args, u2, c->partials, c->nparts_reduce, c->stream))
+                rc = fail(c, DG_ERR_DEVICE, "heat launch");
+            nparts = c->nparts_reduce;
+        }
+        if (dg::launch_finalize(c->prec, c->args, c->sc, c->partials, nparts, c->cfg.cfl,
                                 c->cfg.dt_nan_fallback, 1, c->stream))
             rc = fail(c, DG_ERR_DEVICE, "finalize launch");
         if (hipEventRecord(ev[2 * s + 2], c->stream) != hipSuccess) rc = DG_ERR_DEVICE;

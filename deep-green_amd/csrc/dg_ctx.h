@@ -30,6 +30,8 @@ struct dg_ctx {
     bool owns_stream = true;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // dg_run replay, by state buffer parity
     int graph_steps = 0;
+    int heat_mask = 0;        // conducting lower walls (bit 0 x-, 1 y-, 2 z-)
+    double heat_t_out = 273.0, heat_v_out = 0.0, heat_rs = 287.0;
     bool have_state = false;
     bool sums_stale = true;   // the step kernel does not accumulate sums
     std::string err;

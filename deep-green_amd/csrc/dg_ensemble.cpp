@@ -296,6 +296,11 @@ int dg_ensemble_create(const dg_config *cfgs, int count, int grid, dg_ensemble *
         dg_ensemble_destroy(e);
         return rc;
     }
+    for (dg_ctx *c : e->inst)
+        if (c->heat_mask) {   // the persistent step kernel has no heat source
+            dg_ensemble_destroy(e);
+            return DG_ERR_INVALID;
+        }
     dg_ctx *f = e->inst[0];
     e->prec = f->prec;
     e->fast = f->fast;

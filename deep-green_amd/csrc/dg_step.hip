@@ -36,6 +36,8 @@
 // for bit.  FAST = true is the same scheme reassociated around FMAs and
 // hardware reciprocals.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <math.h>
 #include <stdint.h>
 
@@ -1092,6 +1094,57 @@ residual_kernel(long long n3, const Fields<const float> Ut, const Fields<const f
 
 __global__ void set_dt_kernel(Scalars *sc, double dt) { sc->dt = dt; }
 
+// ---------------------------------------------------- conducting walls --
+// Heat through conducting lower walls (the CUDA variant's source,
+// src/cu/include/kernels.h:561-661): the first interior cell along a
+// conducting lower wall gains dt ds (ds Q/V) of energy, Q = (T_out - t_in) S /
+// (d/kappa + 1/hc), t_in = p / (RS rho) of the wall ghost (the mirror of the
+// cell itself) at the start of the step, S = ds^2, V = ds^3, d 0.008, kappa
+// 0.2, hc = 10 - v_out + 10 sqrt(v_out).  Added after the step, axes in the
+// order x, y, z; one thread per cell of the union of the wall layers.
+struct HeatArgs { double t_out, v_out, rs; int mask; };
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+heat_kernel(const StepArgs A, const Fields<const T> Uold, const Fields<T> Unew, const Scalars *__restrict__ sc,
+            const HeatArgs H)
+{
+    if (!sc->active) return;
+    const T ds = T(A.ds), gm1 = T(A.gamma) - T(1.0);
+    const T S = ds * ds, V = ds * ds * ds;
+    const T vout = T(H.v_out);
+    const T hc = T(10.0) - vout + T(10.0) * dsqrt<T>(vout);
+    const T dk = T(0.008) / T(0.2);
+    const T dt = T(sc->dt);
+    const int n0 = A.n[0], n1 = A.n[1], n2 = A.n[2];
+    // cells of the union: layer x (i = 0), then y (j = 0, i > 0 if x is a wall layer), then z
+    const long long lx = (H.mask & 1) ? (long long)n1 * n2 : 0;
+    const long long ly = (H.mask & 2) ? (long long)n0 * n2 : 0;
+    const long long lz = (H.mask & 4) ? (long long)n0 * n1 : 0;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < lx + ly + lz;
+         e += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        if (e < lx) { i = 0; j = (int)(e / n2); k = (int)(e % n2); }
+        else if (e < lx + ly) { const long long r = e - lx; j = 0; i = (int)(r / n2); k = (int)(r % n2); }
+        else { const long long r = e - lx - ly; k = 0; i = (int)(r / n1); j = (int)(r % n1); }
+        // the cell is handled by the first layer (x, y, z order) that contains it
+        if (e >= lx && (H.mask & 1) && i == 0) continue;
+        if (e >= lx + ly && (H.mask & 2) && j == 0) continue;
+        const long long c = ((long long)i * n1 + j) * n2 + k;
+        T u[5], w[5];
+        for (int f = 0; f < 5; ++f) u[f] = Uold.f[f][c];
+        to_prim<T, false>(u, V, T(1) / V, gm1, w);
+        const T t_in = w[4] / (T(H.rs) * w[0]);
+        const T Q = (T(H.t_out) - t_in) * (S / (dk + T(1.0) / hc));
+        const T dE = dt * ds * ((Q / V) * ds);
+        T E = Unew.f[4][c];
+        const bool on[3] = {(H.mask & 1) && i == 0, (H.mask & 2) && j == 0, (H.mask & 4) && k == 0};
+        for (int d = 0; d < 3; ++d)
+            if (on[d]) E += dE;
+        Unew.f[4][c] = E;
+    }
+}
+
 // ---------------------------------------------------------------- launch --
 template <typename T> static Fields<const T> cfields(const void *const p[5])
 {
@@ -1296,6 +1349,21 @@ int launch_residual(long long n3, const void *const ut[5], const void *const up[
 int launch_set_dt(Scalars *sc, double dt, hipStream_t s)
 {
     hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, s, sc, dt);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_heat(int prec, const StepArgs &a, const void *const uold[5], void *const unew[5], const Scalars *sc,
+                double t_out, double v_out, double rs, int mask, hipStream_t s)
+{
+    HeatArgs H{t_out, v_out, rs, mask};
+    const long long cells = (long long)a.n[1] * a.n[2] + (long long)a.n[0] * a.n[2] + (long long)a.n[0] * a.n[1];
+    const int blocks = (int)std::min<long long>((cells + 255) / 256, 2048);
+    if (prec == 0)
+        hipLaunchKernelGGL(heat_kernel<double>, dim3(blocks), dim3(256), 0, s, a, cfields<double>(uold),
+                           mfields<double>(unew), sc, H);
+    else
+        hipLaunchKernelGGL(heat_kernel<float>, dim3(blocks), dim3(256), 0, s, a, cfields<float>(uold),
+                           mfields<float>(unew), sc, H);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -20,6 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "libdeepgreen.so")
 
 PERIODIC, OUTFLOW, WALL = 0, 1, 2          # grid.rs:24-28
+CONDUCTING_WALL = 3                        # src/cu/include/constants.h:44
 FP64, FP32, FP32_BULK = 0, 1, 2
 EXACT, FAST = 0, 1
 LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2}
@@ -33,7 +34,7 @@ STATUS = {0: "ok", -1: "invalid argument", -2: "device error", -3: "allocation f
 # every entry point declared in include/deepgreen.h
 EXPORTS = ("dg_config_default", "dg_create", "dg_destroy", "dg_last_error", "dg_set_state",
            "dg_get_state", "dg_step", "dg_run", "dg_reset_clock", "dg_conservation_sums",
-           "dg_next_dt", "dg_device_state", "dg_time_steps", "dg_kernel_info", "dg_device_count",
+           "dg_next_dt", "dg_device_state", "dg_time_steps", "dg_kernel_info", "dg_device_count", "dg_set_heat",
            "dg_device_info",
            "dg_ensemble_create", "dg_ensemble_destroy", "dg_ensemble_last_error", "dg_ensemble_set_state",
            "dg_ensemble_get_state", "dg_ensemble_run", "dg_ensemble_clock", "dg_ensemble_conservation_sums",
@@ -99,6 +100,7 @@ def load_library(path=None):
     L.dg_time_steps.argtypes = [vp, i64, P(d), P(d), P(d)]
     L.dg_kernel_info.argtypes = [vp, P(i32), P(i32), P(i64), P(i32), P(i32)]
     L.dg_device_count.argtypes = [P(i32)]
+    L.dg_set_heat.argtypes = [vp, d, d, d]
     L.dg_ensemble_create.argtypes = [P(DgConfig), i32, i32, P(vp)]
     L.dg_ensemble_destroy.argtypes = [vp]
     L.dg_ensemble_destroy.restype = None
@@ -266,6 +268,10 @@ class Solver:
         dt = ctypes.c_double()
         self._check(self._L.dg_step(self._h, ctypes.byref(dt)), "dg_step")
         return dt.value
+
+    def set_heat(self, t_out_kelvin=273.0, v_out=0.0, rs=287.0):
+        """Constants of the conducting-wall heat source (dg_set_heat)."""
+        self._check(self._L.dg_set_heat(self._h, float(t_out_kelvin), float(v_out), float(rs)), "dg_set_heat")
 
     def step_dt(self, dt):
         """One step with the given dt (the next step uses the CFL dt again)."""

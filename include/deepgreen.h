@@ -35,8 +35,12 @@ typedef enum dg_status {
     DG_ERR_STATE = -5        /* no state set yet */
 } dg_status;
 
-/* BoundaryType, grid.rs:24-28 (same numbering). */
-typedef enum dg_boundary { DG_PERIODIC = 0, DG_OUTFLOW = 1, DG_WALL = 2 } dg_boundary;
+/* BoundaryType, grid.rs:24-28 (same numbering).  DG_CONDUCTING_WALL is the
+ * CUDA variant's BC::CONDUCTING_WALL (src/cu/include/constants.h:44): a wall
+ * for the flow; on a lower face (x-, y-, z-) the first interior cell also
+ * exchanges heat through the wall (src/cu/include/kernels.h:561-661; the
+ * CUDA upper-face branches change only ghost cells).  Constants: dg_set_heat. */
+typedef enum dg_boundary { DG_PERIODIC = 0, DG_OUTFLOW = 1, DG_WALL = 2, DG_CONDUCTING_WALL = 3 } dg_boundary;
 
 typedef enum dg_precision {
     DG_FP64 = 0,      /* state, stencil, dt and sums in double */
@@ -109,6 +113,14 @@ int dg_step(dg_ctx *ctx, double *dt_used);
  * previous calls; *t_out and *steps_out receive the totals. */
 int dg_run(dg_ctx *ctx, int64_t max_steps, double t_end, double *t_out, int64_t *steps_out);
 
+/* Constants of the conducting-wall heat source: outside temperature (K),
+ * outside wind speed v_out (m/s; hc = 10 - v_out + 10 sqrt(v_out)) and the
+ * specific gas constant (defaults 273, 0, 287: src/cu/include/constants.h:63-66).
+ * Each step then adds dt ds (ds Q / V) to the energy of the first interior cell
+ * along a conducting lower wall, Q = (T_out - p/(RS rho)) ds^2 / (0.008/0.2 +
+ * 1/hc), V = ds^3 (p, rho of that cell at the start of the step). */
+int dg_set_heat(dg_ctx *ctx, double t_out_kelvin, double v_out, double rs);
+
 /* Reset the run clock (t = 0, steps = 0) kept by dg_run. */
 int dg_reset_clock(dg_ctx *ctx);
 
@@ -126,8 +138,9 @@ int dg_device_state(dg_ctx *ctx, void *ptrs[5]);
 
 /* Timing harness: runs nsteps plain (non-graph) steps on the context's stream
  * with a HIP event pair around every launch; reports the mean device time of
- * the fused step kernel and of the per-step reduction kernel, and the total
- * wall time of the nsteps steps (also measured with events). */
+ * the fused step kernel and of the per-step dt kernels (finalize; with
+ * conducting walls also the heat and reduction kernels), and the total wall
+ * time of the nsteps steps (also measured with events). */
 int dg_time_steps(dg_ctx *ctx, int64_t nsteps, double *ms_total, double *ms_step_kernel,
                   double *ms_reduce_kernel);
 

@@ -42,7 +42,8 @@ typedef struct dgo_params {
     int32_t nthreads;        /* OpenMP threads for the passes */
     int32_t ref_threading;   /* 1: xtr/flux/add/min serial, as in physics.rs */
     int32_t limiter;         /* DGO_LIM_*; 0 is the reference scheme */
-    int32_t pad_;
+    int32_t heat_mask;       /* conducting lower walls: bit 0 x-, 1 y-, 2 z- (0: none) */
+    double t_out, v_out, rs; /* heat source constants (src/cu/include/constants.h:63-66) */
 } dgo_params;
 
 #define R float

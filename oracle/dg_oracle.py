@@ -34,7 +34,8 @@ class _Params(ctypes.Structure):
         ("g", ctypes.c_double), ("gravity", ctypes.c_int32),
         ("gravity_k_shift", ctypes.c_int32), ("dt_nan_fallback", ctypes.c_int32),
         ("nthreads", ctypes.c_int32), ("ref_threading", ctypes.c_int32),
-        ("limiter", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("limiter", ctypes.c_int32), ("heat_mask", ctypes.c_int32),
+        ("t_out", ctypes.c_double), ("v_out", ctypes.c_double), ("rs", ctypes.c_double),
     ]
 
 
@@ -86,7 +87,7 @@ class Oracle:
 
     def __init__(self, interior, ds, bcs, gamma=1.4, cfl=0.4, g=0.0, gravity=False,
                  precision="f64", gravity_k_shift=0, dt_nan_fallback=True,
-                 nthreads=1, ref_threading=False, limiter="none"):
+                 nthreads=1, ref_threading=False, limiter="none", t_out=273.0, v_out=0.0, rs=287.0):
         self.sfx = {"f32": "f32", "f64": "f64", "float32": "f32", "float64": "f64"}[precision]
         self.dtype = np.float32 if self.sfx == "f32" else np.float64
         self.interior = tuple(int(v) for v in interior)
@@ -94,8 +95,12 @@ class Oracle:
         p = _Params()
         p.nx, p.ny, p.nz = self.shape
         p.nghosts = 2
+        # a conducting wall (3) is a wall for the flow; lower ones carry the
+        # heat source (kernels.h:561-661)
         for i, b in enumerate(bcs):
-            p.bcs[i] = int(b)
+            p.bcs[i] = 2 if int(b) == 3 else int(b)
+        p.heat_mask = sum(1 << (i // 2) for i, b in enumerate(bcs) if int(b) == 3 and i % 2 == 0)
+        p.t_out, p.v_out, p.rs = float(t_out), float(v_out), float(rs)
         p.ds, p.gamma, p.cfl, p.g = float(ds), float(gamma), float(cfl), float(g)
         p.gravity = int(bool(gravity))
         p.gravity_k_shift = int(gravity_k_shift)

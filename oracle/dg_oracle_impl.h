@@ -457,6 +457,44 @@ void FN(dgo_add_fluxes)(CTX *c, R dt)
     }
 }
 
+/* Conducting-wall heat source of the CUDA variant (src/cu/include/kernels.h:
+ * 561-661), restated: for a conducting lower wall along axis d, the energy
+ * flux through the wall face of the first interior cell gains Q/V, with
+ * t_in = p / (RS rho) of the wall ghost (index 1, the mirror of the first
+ * interior cell, from the primitives of the step's start), hc = 10 - v_out +
+ * 10 sqrt(v_out), S = ds^2, V = ds^3, Q = (T_out - t_in) S / (d/kappa +
+ * 1/hc), d = 0.008, kappa = 0.2.  Through the update (F scaled by ds,
+ * U -= dt ds dF; physics.rs:401-405, 539-593) the first interior cell's
+ * energy gains dt ds (ds Q/V); added after add_fluxes, axes in the order x,
+ * y, z.  The CUDA code's upper-wall branches act on ghost cells only and
+ * change no interior cell, so only lower walls carry the source.
+ * PARITY UNPINNED: the CUDA build cannot run here. */
+void FN(dgo_heat)(CTX *c, R dt)
+{
+    if (!c->p.heat_mask) return;
+    const R ds = (R)c->p.ds;
+    const R S = ds * ds, V = ds * ds * ds;
+    const R vout = (R)c->p.v_out;
+    const R hc = (R)10.0 - vout + (R)10.0 * RSQRT(vout);
+    const R dk = (R)0.008 / (R)0.2;
+    const int64_t n[3] = {c->p.nx, c->p.ny, c->p.nz};
+    for (int d = 0; d < 3; ++d) {
+        if (!(c->p.heat_mask & (1 << d))) continue;
+        for (int64_t i = 2; i < n[0] - 2; ++i)
+            for (int64_t j = 2; j < n[1] - 2; ++j)
+                for (int64_t k = 2; k < n[2] - 2; ++k) {
+                    const int64_t q[3] = {i, j, k};
+                    if (q[d] != 2) continue;
+                    int64_t gq[3] = {i, j, k};
+                    gq[d] = 1;   /* the wall ghost */
+                    const int64_t gi = I3(c, gq[0], gq[1], gq[2]);
+                    const R t_in = c->prim[4][gi] / ((R)c->p.rs * c->prim[0][gi]);
+                    const R Q = ((R)c->p.t_out - t_in) * (S / (dk + (R)1.0 / hc));
+                    c->cons[4][I3(c, i, j, k)] += dt * ds * ((Q / V) * ds);
+                }
+    }
+}
+
 /* compute_step (physics.rs:595-681).  The incoming *dt is ignored. */
 void FN(dgo_compute_step)(CTX *c, R *dt)
 {
@@ -470,6 +508,7 @@ void FN(dgo_compute_step)(CTX *c, R *dt)
     FN(dgo_calc_fluxes)(c);
     FN(dgo_update_fluxes_ghost_cells)(c);
     FN(dgo_add_fluxes)(c, *dt);
+    FN(dgo_heat)(c, *dt);
 }
 
 /* compute (main.rs:270-300) without the HDF5 dump: calc_conserved, then step

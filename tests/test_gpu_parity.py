@@ -244,3 +244,25 @@ def test_blast_fp64_minmod_64cubed_matches_oracle():
     rel_l1 = np.abs(U - Uo).reshape(5, -1).sum(1) / np.abs(Uo).reshape(5, -1).sum(1)
     assert np.all(rel_l1 <= 1e-12), rel_l1
     np.testing.assert_array_equal(U, Uo)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_conducting_walls_match_oracle(precision):
+    """Conducting lower walls along x and y (heat source), a wall, an outflow
+    face: bit for bit the oracle over 40 steps (graph replays included)."""
+    from deepgreen import Simulation, Solver
+    interior = (20, 18, 70)
+    w, ds = cases.smooth_random(interior, seed=32, amp_v=0.1, p_base=100.0)
+    bcs = [3, 2, 3, 1, 2, 2]
+    prm = dict(gamma=5.0 / 3.0, cfl=0.2, g=0.0)
+    _say(f"  conducting walls {precision}")
+    with Solver(Simulation(*interior, ds, bcs, precision=precision, **prm)) as s:
+        s.set_heat(300.0, 4.0, 287.0)
+        s.set_primitives(w)
+        s.run(40)
+        U = np.stack([np.array(s.interior(a)) for a in s.get_state()])
+    o = Oracle(interior, ds, bcs, precision="f64" if precision == "fp64" else "f32", nthreads=8, t_out=300.0,
+               v_out=4.0, rs=287.0, **prm)
+    o.set_primitives(w)
+    o.compute(1e30, 40)
+    np.testing.assert_array_equal(U, np.stack([np.array(o.interior_view(a)) for a in o.cons]))

@@ -173,3 +173,26 @@ def test_emulated_ensemble_retile_invariance(emu_lib):
             ref = s.get_state()
         for a, b in zip(got[i], ref):
             np.testing.assert_array_equal(a[2:-2, 2:-2, 2:-2], b[2:-2, 2:-2, 2:-2])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_emulated_conducting_walls_match_oracle(emu_lib, precision):
+    """Conducting lower walls along x and z (the heat source of the CUDA
+    variant), plain walls elsewhere: bit for bit the oracle, over steps that
+    include a captured graph."""
+    import deepgreen
+    interior = (6, 5, 7)
+    w, ds = cases.smooth_random(interior, seed=31, amp_v=0.1, p_base=100.0)
+    bcs = [3, 2, 2, 1, 3, 2]
+    prm = dict(gamma=5.0 / 3.0, cfl=0.2, g=0.0)
+    sim = deepgreen.Simulation(*interior, ds, bcs, precision=precision, tile_y=4, **prm)
+    with deepgreen.Solver(sim, lib=emu_lib) as s:
+        s.set_heat(300.0, 4.0, 287.0)
+        s.set_primitives(w)
+        s.run(17)
+        U = np.stack([np.array(s.interior(a)) for a in s.get_state()])
+    o = Oracle(interior, ds, bcs, precision="f64" if precision == "fp64" else "f32", t_out=300.0, v_out=4.0,
+               rs=287.0, **prm)
+    o.set_primitives(w)
+    o.compute(1e30, 17)
+    np.testing.assert_array_equal(U, np.stack([np.array(o.interior_view(a)) for a in o.cons]))

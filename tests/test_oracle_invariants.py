@@ -254,3 +254,35 @@ def test_limited_scheme_blast_stays_positive(lim):
     o2 = Oracle((n, n, n), ds, [PERIODIC] * 6, precision="f64", nthreads=4)
     U2 = _run(o2, w, 5)
     assert not all(np.isfinite(u).all() for u in U2)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_conducting_wall_heat_closed_form(axis):
+    """A uniform gas at rest in a box whose lower wall along `axis` is
+    conducting: the flow stays at rest (a fixed point of the walled scheme)
+    and, after one step, only the first interior layer along that axis has
+    gained energy, by dt ds^2 Q / V with Q = (T_out - p/(RS rho)) ds^2 /
+    (0.008/0.2 + 1/hc), hc = 10 - v + 10 sqrt(v) (double-precision closed form
+    of the CUDA variant's source, src/cu/include/kernels.h:561-661)."""
+    n, ds = (6, 5, 7), 0.1
+    rho, p, t_out, v_out, rs = 1.293, 101325.0, 300.0, 4.0, 287.0
+    bcs = [2] * 6
+    bcs[2 * axis] = 3
+    o = Oracle(n, ds, bcs, precision="f64", gamma=5.0 / 3.0, cfl=0.2, g=0.0, t_out=t_out, v_out=v_out, rs=rs)
+    shape = tuple(k + 4 for k in n)
+    o.set_primitives([np.full(shape, rho), np.zeros(shape), np.zeros(shape), np.zeros(shape), np.full(shape, p)])
+    o.calc_conserved()
+    E0 = np.array(o.interior_view(o.cons[4])).copy()
+    U0 = [np.array(o.interior_view(a)).copy() for a in o.cons[:4]]
+    dt = o.steps(1)[0]
+    E1 = np.array(o.interior_view(o.cons[4]))
+    hc = 10.0 - v_out + 10.0 * np.sqrt(v_out)
+    Q = (t_out - p / (rs * rho)) * ds * ds / (0.008 / 0.2 + 1.0 / hc)
+    expect = E0.copy()
+    sl = [slice(None)] * 3
+    sl[axis] = 0
+    expect[tuple(sl)] += dt * ds * ds * Q / ds ** 3
+    np.testing.assert_allclose(E1, expect, rtol=1e-12, atol=0)
+    assert np.abs(E1 - E0).max() > 0
+    for a, b in zip(U0, o.cons[:4]):
+        np.testing.assert_allclose(np.array(o.interior_view(b)), a, rtol=0, atol=1e-10 * rho * ds ** 3)
