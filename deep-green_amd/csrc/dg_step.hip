@@ -429,9 +429,8 @@ struct LoadPlan {
     static constexpr int NT = TY * TZ;
     static constexpr int NREG = (TY + 4) * RZ;
     static constexpr int NS = (NREG + NT - 1) / NT;
-    int lds[NS];
+    int lds[NS];    // flat LDS index | flips << 12 (bit 13: y, bit 14: z); -1: none
     int off[NS];
-    int flip[NS];   // bit 1: y, bit 2: z
 };
 
 template <int TY>
@@ -441,7 +440,7 @@ __device__ __forceinline__ void make_plan(LoadPlan<TY> &P, int tid, int y0, int 
 #pragma unroll
     for (int q = 0; q < LoadPlan<TY>::NS; ++q) {
         const int c = tid + q * NT;
-        if (c >= NREG) { P.lds[q] = -1; P.off[q] = 0; P.flip[q] = 0; continue; }
+        if (c >= NREG) { P.lds[q] = -1; P.off[q] = 0; continue; }
         int ry, rz;
         if (c < NT) { ry = (c >> 6) + 2; rz = (c & 63) + 2; }   // the thread's own cell first
         else {
@@ -459,36 +458,63 @@ __device__ __forceinline__ void make_plan(LoadPlan<TY> &P, int tid, int y0, int 
         }
         const AxisMap my = map_axis(y0 + ry - 2, A.n[1], A.bc[2], A.bc[3]);
         const AxisMap mz = map_axis(z0 + rz - 2, A.n[2], A.bc[4], A.bc[5]);
-        P.lds[q] = ry * RZ + rz;
+        P.lds[q] = (ry * RZ + rz) | (my.flip ? 1 << 13 : 0) | (mz.flip ? 1 << 14 : 0);
         P.off[q] = my.idx * A.n[2] + mz.idx;
-        P.flip[q] = (my.flip ? 2 : 0) | (mz.flip ? 4 : 0);
     }
 }
 
 // Load plane xg's primitives into ring slot `slot` (calc_primitives on the
-// mapped cells; ghost positions take the boundary maps).
-template <typename T, int TY, bool FAST>
-__device__ __forceinline__ void load_plane(StepLDS<T, TY> &L, int slot, int xg, const LoadPlan<TY> &P,
-                                           const StepArgs &A, const Fields<const T> &U, const Consts<T> &C)
+// mapped cells; ghost positions take the boundary maps).  Split in two so
+// the global loads can be issued one phase before their values are needed.
+template <typename T, int TY>
+struct PlaneLoad {
+    T u[LoadPlan<TY>::NS][5];
+    int xflip;
+};
+
+template <typename T, int TY>
+__device__ __forceinline__ void issue_plane(PlaneLoad<T, TY> &Q, int xg, const LoadPlan<TY> &P, const StepArgs &A,
+                                            const Fields<const T> &U)
 {
-    constexpr int FS = StepLDS<T, TY>::FS, PS = StepLDS<T, TY>::PS;
     const AxisMap mx = map_axis(xg, A.n[0], A.bc[0], A.bc[1]);
     const uint32_t plane = (uint32_t)mx.idx * (uint32_t)A.n[1] * (uint32_t)A.n[2];
-    T *Ws = L.W + slot * PS;
+    Q.xflip = mx.flip;
 #pragma unroll
     for (int q = 0; q < LoadPlan<TY>::NS; ++q) {
         if (P.lds[q] < 0) continue;
         const uint32_t idx = plane + (uint32_t)P.off[q];
-        T u[5], w[5];
 #pragma unroll
-        for (int f = 0; f < 5; ++f) u[f] = at(U.f[f], idx);
-        to_prim<T, FAST>(u, C.V, C.invV, C.gm1, w);
-        if (mx.flip) w[1] = -w[1];
-        if (P.flip[q] & 2) w[2] = -w[2];
-        if (P.flip[q] & 4) w[3] = -w[3];
-#pragma unroll
-        for (int f = 0; f < 5; ++f) Ws[f * FS + P.lds[q]] = w[f];
+        for (int f = 0; f < 5; ++f) Q.u[q][f] = at(U.f[f], idx);
     }
+}
+
+template <typename T, int TY, bool FAST>
+__device__ __forceinline__ void finish_plane(StepLDS<T, TY> &L, int slot, const PlaneLoad<T, TY> &Q,
+                                             const LoadPlan<TY> &P, const Consts<T> &C)
+{
+    constexpr int FS = StepLDS<T, TY>::FS, PS = StepLDS<T, TY>::PS;
+    T *Ws = L.W + slot * PS;
+#pragma unroll
+    for (int q = 0; q < LoadPlan<TY>::NS; ++q) {
+        if (P.lds[q] < 0) continue;
+        T w[5];
+        to_prim<T, FAST>(Q.u[q], C.V, C.invV, C.gm1, w);
+        if (Q.xflip) w[1] = -w[1];
+        if (P.lds[q] & (1 << 13)) w[2] = -w[2];
+        if (P.lds[q] & (1 << 14)) w[3] = -w[3];
+        const int li = P.lds[q] & 0xfff;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) Ws[f * FS + li] = w[f];
+    }
+}
+
+template <typename T, int TY, bool FAST>
+__device__ __forceinline__ void load_plane(StepLDS<T, TY> &L, int slot, int xg, const LoadPlan<TY> &P,
+                                           const StepArgs &A, const Fields<const T> &U, const Consts<T> &C)
+{
+    PlaneLoad<T, TY> Q;
+    issue_plane<T, TY>(Q, xg, P, A, U);
+    finish_plane<T, TY, FAST>(L, slot, Q, P, C);
 }
 
 // Deterministic block reduction of (max den, nan flag, 5 sums) into partials.
@@ -594,15 +620,17 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
     double dmaxden = 0.0;
     int nanf = 0;
 
+    // the ring holds planes t-1, t, t+1 when plane t's step starts; plane
+    // t+2 is loaded during the flux phase of plane t (its slot held t-1,
+    // whose last reader is phase B of plane t)
     load_plane<T, TY, FAST>(L, ring(x0 - 2), x0 - 2, plan, A, U, C);
     load_plane<T, TY, FAST>(L, ring(x0 - 1), x0 - 1, plan, A, U, C);
+    load_plane<T, TY, FAST>(L, ring(x0), x0, plan, A, U, C);
+    __syncthreads();
 
     for (int t = x0 - 1; t <= xend; ++t) {
         const bool full = t >= x0 && t < xend;
         const int sm = ring(t - 1), s0 = ring(t), sp = ring(t + 1);
-        // ---- A: primitives of plane t+1 into the ring
-        load_plane<T, TY, FAST>(L, sp, t + 1, plan, A, U, C);
-        __syncthreads();
 
         // ---- B: halo face states (waves 0-2), then the own cell
         if (full && (ty < 2 || (ty == 2 && tz < 2 * TY))) {
@@ -642,7 +670,12 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
         }
         __syncthreads();
 
-        // ---- C: fluxes through the own cell's +x (of plane t-1), +y, +z faces
+        // ---- C: fluxes through the own cell's +x (of plane t-1), +y, +z faces;
+        // the loads of plane t+2 are issued first so their latency overlaps
+        // the flux arithmetic
+        const bool pre = t + 2 <= xend + 1;
+        PlaneLoad<T, TY> next;
+        if (pre) issue_plane<T, TY>(next, t + 2, plan, A, U);
         T fxu[5] = {T(0), T(0), T(0), T(0), T(0)};  // face (t-1, t) along x
         const bool xbnd = (t == 0 && A.bc[0] == BC_WALL) || (t == nx && A.bc[1] != BC_PERIODIC);
         if (t >= x0 && !xbnd) {
@@ -702,6 +735,15 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
         }
 #pragma unroll
         for (int f = 0; f < 5; ++f) { xprev[f] = xtr[f]; gxprev[f] = g[f * 3 + 0]; }
+        if (pre) finish_plane<T, TY, FAST>(L, ring(t + 2), next, plan, C);
+        const int xc = t - 1;
+        const bool upd = xc >= x0 && xc < xend && valid;
+        const uint32_t uidx = (uint32_t)xc * plane_sz + col;
+        T uold[5];
+        if (upd) {
+#pragma unroll
+            for (int f = 0; f < 5; ++f) uold[f] = at(U.f[f], uidx);
+        }
         __syncthreads();
 
         // ---- D: lower y/z fluxes, finish cell t-1, carry plane t
@@ -722,22 +764,25 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
                 for (int f = 0; f < 5; ++f) fzl_n[f] = zlo_out ? fzu_n[f] : fzl_n[f];
             }
         }
-        const int xc = t - 1;
-        if (xc >= x0 && xc < xend && valid) {
-            if (xc == 0 && A.bc[0] == BC_OUTFLOW) {
+        if (upd) {
+            // outflow at the lower x face: the first cell's lower face copies
+            // its upper face (boundaries.rs:107-111 on fluxes)
+            const bool xlo_out = xc == 0 && A.bc[0] == BC_OUTFLOW;
+            if constexpr (!FAST) {
+                if (xlo_out) {
 #pragma unroll
-                for (int f = 0; f < 5; ++f) fxlo[f] = fxu[f];
+                    for (int f = 0; f < 5; ++f) fxlo[f] = fxu[f];
+                }
             }
-            const uint32_t idx = (uint32_t)xc * plane_sz + col;
             T un[5];
 #pragma unroll
             for (int f = 0; f < 5; ++f) {
-                const T u = at(U.f[f], idx);
+                const T u = uold[f];
                 T acc;
                 if constexpr (!FAST) acc = ((((fxu[f] - fxlo[f]) + fyu[f]) - fyl[f]) + fzu[f]) - fzl[f];
-                else acc = (fxu[f] - fxlo[f]) + fyu[f];   // fyu holds the y/z partial sum
+                else acc = xlo_out ? fyu[f] : fxu[f] + fyu[f];   // fyu: y/z sum minus the lower x face
                 un[f] = FAST ? dfma<T>(-C.dtds, acc, u) : u - C.dtds * acc;
-                at(Vout.f[f], idx) = un[f];
+                at(Vout.f[f], uidx) = un[f];
             }
             T w[5];
             to_prim<T, FAST>(un, C.V, C.invV, C.gm1, w);
@@ -752,13 +797,15 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
             }
         }
         if (full) {
+            const bool xlo_next = t == 0 && A.bc[0] == BC_OUTFLOW;
 #pragma unroll
             for (int f = 0; f < 5; ++f) {
-                fxlo[f] = fxu[f];
                 if constexpr (!FAST) {
+                    fxlo[f] = fxu[f];
                     fyu[f] = fyu_n[f]; fyl[f] = fyl_n[f]; fzu[f] = fzu_n[f]; fzl[f] = fzl_n[f];
                 } else {
-                    fyu[f] = ((fyu_n[f] - fyl_n[f]) + fzu_n[f]) - fzl_n[f];
+                    const T yz = ((fyu_n[f] - fyl_n[f]) + fzu_n[f]) - fzl_n[f];
+                    fyu[f] = xlo_next ? yz : yz - fxu[f];
                 }
             }
         }
