@@ -66,25 +66,30 @@ template <> __device__ __forceinline__ float dsqrt_fast<float>(float x) { return
 template <> __device__ __forceinline__ double dsqrt_fast<double>(double x) { return sqrt(x); }
 
 // Whole-wavefront lane shifts (DPP wave_shl:1 / wave_shr:1): lane i receives
-// lane i+1 (shl) or lane i-1 (shr); the lane without a source gets 0.
-__device__ __forceinline__ int dpp_from_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
-__device__ __forceinline__ int dpp_from_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
-template <typename T> __device__ __forceinline__ T from_next_lane(T v);
-template <typename T> __device__ __forceinline__ T from_prev_lane(T v);
-template <> __device__ __forceinline__ float from_next_lane<float>(float v)
-{ return __builtin_bit_cast(float, dpp_from_next(__builtin_bit_cast(int, v))); }
-template <> __device__ __forceinline__ float from_prev_lane<float>(float v)
-{ return __builtin_bit_cast(float, dpp_from_prev(__builtin_bit_cast(int, v))); }
-template <> __device__ __forceinline__ double from_next_lane<double>(double v)
+// lane i+1 (shl) or lane i-1 (shr); the lane without a source (63 for shl, 0
+// for shr) keeps `edge` (bound_ctrl off: the destination is not written).
+__device__ __forceinline__ int dpp_from_next(int edge, int v)
+{ return __builtin_amdgcn_update_dpp(edge, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_from_prev(int edge, int v)
+{ return __builtin_amdgcn_update_dpp(edge, v, 0x138, 0xf, 0xf, false); }
+template <typename T> __device__ __forceinline__ T from_next_lane(T v, T edge);
+template <typename T> __device__ __forceinline__ T from_prev_lane(T v, T edge);
+template <> __device__ __forceinline__ float from_next_lane<float>(float v, float edge)
+{ return __builtin_bit_cast(float, dpp_from_next(__builtin_bit_cast(int, edge), __builtin_bit_cast(int, v))); }
+template <> __device__ __forceinline__ float from_prev_lane<float>(float v, float edge)
+{ return __builtin_bit_cast(float, dpp_from_prev(__builtin_bit_cast(int, edge), __builtin_bit_cast(int, v))); }
+template <> __device__ __forceinline__ double from_next_lane<double>(double v, double edge)
 {
-    const long long b = __builtin_bit_cast(long long, v);
-    const int lo = dpp_from_next((int)(b & 0xffffffffLL)), hi = dpp_from_next((int)(b >> 32));
+    const long long b = __builtin_bit_cast(long long, v), e = __builtin_bit_cast(long long, edge);
+    const int lo = dpp_from_next((int)(e & 0xffffffffLL), (int)(b & 0xffffffffLL));
+    const int hi = dpp_from_next((int)(e >> 32), (int)(b >> 32));
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
-template <> __device__ __forceinline__ double from_prev_lane<double>(double v)
+template <> __device__ __forceinline__ double from_prev_lane<double>(double v, double edge)
 {
-    const long long b = __builtin_bit_cast(long long, v);
-    const int lo = dpp_from_prev((int)(b & 0xffffffffLL)), hi = dpp_from_prev((int)(b >> 32));
+    const long long b = __builtin_bit_cast(long long, v), e = __builtin_bit_cast(long long, edge);
+    const int lo = dpp_from_prev((int)(e & 0xffffffffLL), (int)(b & 0xffffffffLL));
+    const int hi = dpp_from_prev((int)(e >> 32), (int)(b >> 32));
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -273,14 +278,14 @@ struct StepLDS {
     static constexpr int FS = (TY + 4) * RZ;  // one field of one plane
     static constexpr int PS = 5 * FS;         // one plane
     T W[3 * PS];             // primitives ring [plane t mod 3][field][row][col]
-    T Ym[5][TY + 1][TZ];     // -y face state of rows 0..TY
+    // Ym[.][r]: -y face state of row r (r = 0..TY) until the flux phase, in
+    // which the one thread that reads it replaces it by the flux through
+    // face (r-1, r) (r = 0: the tile's lower y face)
+    T Ym[5][TY + 1][TZ];
     T Yp[5][TZ];             // +y face state of row -1
-    T Yb[5][TZ];             // flux through the tile's lower y face
-    T FY[5][TY][TZ];         // flux through face (row, row+1)
     T Zp[5][TY];             // +z face state of column -1
     T Zm[5][TY];             // -z face state of column 64
-    T Z0[5][TY];             // -z face state of column 0
-    T FZb[5][TY];            // flux through the tile's lower z face
+    T Z0[5][TY];             // -z face state of column 0, then the flux through the tile's lower z face
 };
 
 // Slope of one field along one axis from the primitives m (at -1), c, p (at
@@ -707,13 +712,10 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
             for (int f = 0; f < 5; ++f) l[f] = L.Ym[f][ty + 1][tz];
             face_flux<T, FAST>(l, r, 1, C, false, T(0), fyu_n);
 #pragma unroll
-            for (int f = 0; f < 5; ++f) L.FY[f][ty][tz] = fyu_n[f];
+            for (int f = 0; f < 5; ++f) L.Ym[f][ty + 1][tz] = fyu_n[f];   // read above by this thread only
             face_state<T, FAST>(xtr, g, 2, true, C, r);
 #pragma unroll
-            for (int f = 0; f < 5; ++f) {
-                const T nb = from_next_lane<T>(szm[f]);
-                l[f] = tz == TZ - 1 ? L.Zm[f][ty] : nb;
-            }
+            for (int f = 0; f < 5; ++f) l[f] = from_next_lane<T>(szm[f], L.Zm[f][ty]);
             face_flux<T, FAST>(l, r, 2, C, zgrav, hz, fzu_n);
             // the tile's lower y faces (wave 3) and lower z faces (wave 0):
             // waves on the SIMDs that evaluated no halo row in phase B
@@ -723,14 +725,14 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
                 for (int f = 0; f < 5; ++f) { rb[f] = L.Yp[f][tz]; lb[f] = L.Ym[f][0][tz]; }
                 face_flux<T, FAST>(lb, rb, 1, C, false, T(0), Fb);
 #pragma unroll
-                for (int f = 0; f < 5; ++f) L.Yb[f][tz] = Fb[f];
+                for (int f = 0; f < 5; ++f) L.Ym[f][0][tz] = Fb[f];
             } else if (ty == 0 && tz < TY) {
                 T rb[5], lb[5], Fb[5];
 #pragma unroll
                 for (int f = 0; f < 5; ++f) { rb[f] = L.Zp[f][tz]; lb[f] = L.Z0[f][tz]; }
                 face_flux<T, FAST>(lb, rb, 2, C, zgrav, hzb, Fb);
 #pragma unroll
-                for (int f = 0; f < 5; ++f) L.FZb[f][tz] = Fb[f];
+                for (int f = 0; f < 5; ++f) L.Z0[f][tz] = Fb[f];
             }
         }
 #pragma unroll
@@ -751,9 +753,8 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
         if (full) {
 #pragma unroll
             for (int f = 0; f < 5; ++f) {
-                fyl_n[f] = ty == 0 ? L.Yb[f][tz] : L.FY[f][ty - 1][tz];
-                const T up = from_prev_lane<T>(fzu_n[f]);
-                fzl_n[f] = tz == 0 ? L.FZb[f][ty] : up;
+                fyl_n[f] = L.Ym[f][ty][tz];
+                fzl_n[f] = from_prev_lane<T>(fzu_n[f], L.Z0[f][ty]);
             }
             if (ylo_out) {                           // boundaries.rs:137-141 on fluxes
 #pragma unroll
