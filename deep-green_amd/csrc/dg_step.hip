@@ -38,6 +38,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <math.h>
 #include <stdint.h>
 
@@ -425,8 +426,6 @@ __device__ __forceinline__ T grav_h(int z, int nz, T ds)
     return (T(nzg) - T(2.0) * T(2)) * ds - (T(kg) - T(2.0)) * ds;
 }
 
-__device__ __forceinline__ int ring(int x) { return (x + 3) % 3; }  // x >= -3
-
 // Region cells (tile plus two-cell halo) a thread loads for every plane:
 // flat LDS index, (y, z) offset of the mapped source cell, and flips.
 template <int TY>
@@ -627,15 +626,17 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
 
     // the ring holds planes t-1, t, t+1 when plane t's step starts; plane
     // t+2 is loaded during the flux phase of plane t (its slot held t-1,
-    // whose last reader is phase B of plane t)
-    load_plane<T, TY, FAST>(L, ring(x0 - 2), x0 - 2, plan, A, U, C);
-    load_plane<T, TY, FAST>(L, ring(x0 - 1), x0 - 1, plan, A, U, C);
-    load_plane<T, TY, FAST>(L, ring(x0), x0, plan, A, U, C);
+    // whose last reader is phase B of plane t).  Plane t sits in slot
+    // (t - x0 + 1) mod 3, so the march is unrolled by three with
+    // compile-time slots.
+    load_plane<T, TY, FAST>(L, 2, x0 - 2, plan, A, U, C);
+    load_plane<T, TY, FAST>(L, 0, x0 - 1, plan, A, U, C);
+    load_plane<T, TY, FAST>(L, 1, x0, plan, A, U, C);
     __syncthreads();
 
-    for (int t = x0 - 1; t <= xend; ++t) {
+    auto plane_step = [&](auto slot, const int t) __attribute__((always_inline)) {
+        constexpr int s0 = decltype(slot)::value, sm = (s0 + 2) % 3, sp = (s0 + 1) % 3;
         const bool full = t >= x0 && t < xend;
-        const int sm = ring(t - 1), s0 = ring(t), sp = ring(t + 1);
 
         // ---- B: halo face states (waves 0-2), then the own cell
         if (full && (ty < 2 || (ty == 2 && tz < 2 * TY))) {
@@ -737,7 +738,7 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
         }
 #pragma unroll
         for (int f = 0; f < 5; ++f) { xprev[f] = xtr[f]; gxprev[f] = g[f * 3 + 0]; }
-        if (pre) finish_plane<T, TY, FAST>(L, ring(t + 2), next, plan, C);
+        if (pre) finish_plane<T, TY, FAST>(L, sm, next, plan, C);
         const int xc = t - 1;
         const bool upd = xc >= x0 && xc < xend && valid;
         const uint32_t uidx = (uint32_t)xc * plane_sz + col;
@@ -810,6 +811,14 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
                 }
             }
         }
+    };
+    for (int t = x0 - 1;;) {
+        plane_step(std::integral_constant<int, 0>{}, t);
+        if (++t > xend) break;
+        plane_step(std::integral_constant<int, 1>{}, t);
+        if (++t > xend) break;
+        plane_step(std::integral_constant<int, 2>{}, t);
+        if (++t > xend) break;
     }
 
     if constexpr (!BULK) dmaxden = (double)maxden;
