@@ -273,12 +273,20 @@ struct Consts {
     T ds, gamma, gm1, igm1, V, invV, two_ds, inv_two_ds, half_ds, half_dt, dtds, G;
 };
 
+// Two-element vector for 8-byte (float) / 16-byte (double) LDS accesses.
+template <typename T> using vec2 = T __attribute__((ext_vector_type(2)));
+
 // LDS layout of the step kernel.
 template <typename T, int TY>
 struct StepLDS {
-    static constexpr int FS = (TY + 4) * RZ;  // one field of one plane
-    static constexpr int PS = 5 * FS;         // one plane
-    T W[3 * PS];             // primitives ring [plane t mod 3][field][row][col]
+    // The primitives ring is cell-major: a cell's 5 primitives and (fast
+    // math) its inverse density sit in 6 consecutive words, read as three
+    // pairs; every neighbour of a cell, in all three ring slots, is then
+    // within the DS offset range of one address register.
+    static constexpr int NC = (TY + 4) * RZ;  // region cells of one plane
+    static constexpr int CS = 6;              // words per cell
+    static constexpr int PS = NC * CS;        // one plane
+    alignas(16) T W[3 * PS];  // [plane slot][row][col][rho, vx, vy, vz, p, 1/rho]
     // Ym[.][r]: -y face state of row r (r = 0..TY) until the flux phase, in
     // which the one thread that reads it replaces it by the flux through
     // face (r-1, r) (r = 0: the tile's lower y face)
@@ -310,7 +318,7 @@ __device__ __forceinline__ T slope(T m, T c, T p, const Consts<T> &C)
 // whose slopes and extrapolated state are used; for a wall ghost along y
 // (mask bit 0) or z (bit 1) the normal velocity of xtr flips and the normal
 // slope is evaluated at the ghost's own position ry_wall / rz_wall.
-struct EvalDesc { int r, ry_wall, rz_wall, mask; };
+struct EvalDesc { int r, ry_wall, rz_wall, mask, walls; };   // walls: some lane of the wave has mask != 0
 
 __device__ __forceinline__ EvalDesc make_desc(int py, int pz, int y0, int z0, const StepArgs &A, int ty_max)
 {
@@ -319,6 +327,7 @@ __device__ __forceinline__ EvalDesc make_desc(int py, int pz, int y0, int z0, co
     EvalDesc d;
     d.r = (qy + 2) * RZ + (qz + 2);
     d.mask = 0;
+    d.walls = 1;
     d.ry_wall = d.rz_wall = d.r;
     if (wall_ghost(py, y0, A.n[1], A.bc[2], A.bc[3])) { d.mask |= 1; d.ry_wall = (py + 2) * RZ + (qz + 2); }
     if (wall_ghost(pz, z0, A.n[2], A.bc[4], A.bc[5])) { d.mask |= 2; d.rz_wall = (qy + 2) * RZ + (pz + 2); }
@@ -328,21 +337,45 @@ __device__ __forceinline__ EvalDesc make_desc(int py, int pz, int y0, int z0, co
 // Slopes and Hancock-extrapolated state (extrapolate_primitives_t,
 // physics.rs:217-282) of a cell described by `d`, from ring slots sm, s0, sp.
 // g[f*3+d] is the slope of field f along axis d.
+template <typename T>
+__device__ __forceinline__ void ld_cell(const T *p, T v[6])
+{
+    const vec2<T> *q = reinterpret_cast<const vec2<T> *>(p);
+    const vec2<T> a = q[0], b = q[1], c = q[2];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y;
+}
+// the 5 primitives only (a neighbour's 1/rho is not needed)
+template <typename T>
+__device__ __forceinline__ void ld_prims(const T *p, T v[6])
+{
+    const vec2<T> *q = reinterpret_cast<const vec2<T> *>(p);
+    const vec2<T> a = q[0], b = q[1];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = p[4]; v[5] = T(0);
+}
+
 template <typename T, int TY, bool FAST, int LIM>
 __device__ __forceinline__ void eval_cell(const StepLDS<T, TY> &L, int sm, int s0, int sp, const EvalDesc &d,
                                           const Consts<T> &C, T xtr[5], T g[15])
 {
-    constexpr int FS = StepLDS<T, TY>::FS, PS = StepLDS<T, TY>::PS;
-    const T *Wm = L.W + sm * PS, *W0 = L.W + s0 * PS, *Wp = L.W + sp * PS;
+    constexpr int CS = StepLDS<T, TY>::CS, PS = StepLDS<T, TY>::PS;
+    const T *W0 = L.W + s0 * PS;
+    const T *c0 = W0 + d.r * CS;
+    T cc[6], xm[6], xp[6], ym[6], yp[6], zm[6], zp[6];
+    ld_cell<T>(c0, cc);
+    ld_prims<T>(L.W + sm * PS + d.r * CS, xm);
+    ld_prims<T>(L.W + sp * PS + d.r * CS, xp);
+    ld_prims<T>(c0 - RZ * CS, ym);
+    ld_prims<T>(c0 + RZ * CS, yp);
+    ld_prims<T>(c0 - CS, zm);
+    ld_prims<T>(c0 + CS, zp);
     T w[5];
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
-        const int o = f * FS + d.r;
-        const T c = W0[o];
+        const T c = cc[f];
         w[f] = c;
-        g[f * 3 + 0] = slope<T, FAST, LIM>(Wm[o], c, Wp[o], C);
-        g[f * 3 + 1] = slope<T, FAST, LIM>(W0[o - RZ], c, W0[o + RZ], C);
-        g[f * 3 + 2] = slope<T, FAST, LIM>(W0[o - 1], c, W0[o + 1], C);
+        g[f * 3 + 0] = slope<T, FAST, LIM>(xm[f], c, xp[f], C);
+        g[f * 3 + 1] = slope<T, FAST, LIM>(ym[f], c, yp[f], C);
+        g[f * 3 + 2] = slope<T, FAST, LIM>(zm[f], c, zp[f], C);
     }
     const T r = w[0], ux = w[1], uy = w[2], uz = w[3], p = w[4];
     const T drx = g[0], dry = g[1], drz = g[2];
@@ -359,8 +392,9 @@ __device__ __forceinline__ void eval_cell(const StepLDS<T, TY> &L, int sm, int s
         xtr[3] = uz - hd * (ux * dzx + uy * dzy + uz * dzz + (T(1.0) / r) * dpz);
         xtr[4] = p - hd * (C.gamma * p * (dxx + dyy + dzz) + ux * dpx + uy * dpy + uz * dpz);
     } else {
-        // hd here is dt / 2 / (2 ds): slopes are undivided differences
-        const T ir = drcp<T>(r);
+        // hd here is dt / 2 / (2 ds): slopes are undivided differences;
+        // the ring carries 1/rho
+        const T ir = cc[5];
         const T div = dxx + dyy + dzz;
         xtr[0] = dfma<T>(-hd, dfma<T>(uz, drz, dfma<T>(uy, dry, dfma<T>(ux, drx, r * div))), r);
         xtr[1] = dfma<T>(-hd, dfma<T>(ir, dpx, dfma<T>(uz, dxz, dfma<T>(uy, dxy, ux * dxx))), ux);
@@ -368,20 +402,21 @@ __device__ __forceinline__ void eval_cell(const StepLDS<T, TY> &L, int sm, int s
         xtr[3] = dfma<T>(-hd, dfma<T>(ir, dpz, dfma<T>(uz, dzz, dfma<T>(uy, dzy, ux * dzx))), uz);
         xtr[4] = dfma<T>(-hd, dfma<T>(uz, dpz, dfma<T>(uy, dpy, dfma<T>(ux, dpx, C.gamma * p * div))), p);
     }
+    if (!d.walls) return;   // wave-uniform: no lane evaluates a wall ghost
     if (d.mask & 1) {   // wall ghost along y: mirrored xtr, slope on the ghost
         xtr[2] = -xtr[2];
 #pragma unroll
         for (int f = 0; f < 5; ++f) {
-            const int o = f * FS + d.ry_wall;
-            g[f * 3 + 1] = slope<T, FAST, LIM>(W0[o - RZ], W0[o], W0[o + RZ], C);
+            const int o = d.ry_wall * CS + f;
+            g[f * 3 + 1] = slope<T, FAST, LIM>(W0[o - RZ * CS], W0[o], W0[o + RZ * CS], C);
         }
     }
     if (d.mask & 2) {
         xtr[3] = -xtr[3];
 #pragma unroll
         for (int f = 0; f < 5; ++f) {
-            const int o = f * FS + d.rz_wall;
-            g[f * 3 + 2] = slope<T, FAST, LIM>(W0[o - 1], W0[o], W0[o + 1], C);
+            const int o = d.rz_wall * CS + f;
+            g[f * 3 + 2] = slope<T, FAST, LIM>(W0[o - CS], W0[o], W0[o + CS], C);
         }
     }
 }
@@ -496,19 +531,26 @@ template <typename T, int TY, bool FAST>
 __device__ __forceinline__ void finish_plane(StepLDS<T, TY> &L, int slot, const PlaneLoad<T, TY> &Q,
                                              const LoadPlan<TY> &P, const Consts<T> &C)
 {
-    constexpr int FS = StepLDS<T, TY>::FS, PS = StepLDS<T, TY>::PS;
+    constexpr int CS = StepLDS<T, TY>::CS, PS = StepLDS<T, TY>::PS;
     T *Ws = L.W + slot * PS;
 #pragma unroll
     for (int q = 0; q < LoadPlan<TY>::NS; ++q) {
+        // wall mirrors: rare, so skipped by the whole wave when no lane has one
+        const bool flips = __any(Q.xflip || (P.lds[q] >= 0 && (P.lds[q] & (3 << 13)) != 0));
         if (P.lds[q] < 0) continue;
-        T w[5];
+        T w[6];
         to_prim<T, FAST>(Q.u[q], C.V, C.invV, C.gm1, w);
-        if (Q.xflip) w[1] = -w[1];
-        if (P.lds[q] & (1 << 13)) w[2] = -w[2];
-        if (P.lds[q] & (1 << 14)) w[3] = -w[3];
-        const int li = P.lds[q] & 0xfff;
-#pragma unroll
-        for (int f = 0; f < 5; ++f) Ws[f * FS + li] = w[f];
+        // fast math: 1/rho = V / U_rho for the Hancock step; exact math pads
+        w[5] = FAST ? C.V * drcp<T>(Q.u[q][0]) : T(0);
+        if (flips) {
+            if (Q.xflip) w[1] = -w[1];
+            if (P.lds[q] & (1 << 13)) w[2] = -w[2];
+            if (P.lds[q] & (1 << 14)) w[3] = -w[3];
+        }
+        vec2<T> *o = reinterpret_cast<vec2<T> *>(Ws + (P.lds[q] & 0xfff) * CS);
+        o[0] = vec2<T>{w[0], w[1]};
+        o[1] = vec2<T>{w[2], w[3]};
+        o[2] = vec2<T>{w[4], w[5]};
     }
 }
 
@@ -565,7 +607,8 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
 {
     constexpr int NT = TY * TZ;
     const int tid = threadIdx.x;
-    const int ty = tid >> 6, tz = tid & 63;
+    // the tile row is the wave index: uniform across the wave (scalar register)
+    const int ty = __builtin_amdgcn_readfirstlane(tid >> 6), tz = tid & 63;
     const int nx = A.n[0], ny = A.n[1], nz = A.n[2];
     const int x0 = bx * A.lx, y0 = by * TY, z0 = bz * TZ;
     const int xend = min(x0 + A.lx, nx);
@@ -601,11 +644,13 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
     // plane-independent addressing, evaluated once
     LoadPlan<TY> plan;
     make_plan<TY>(plan, tid, y0, z0, A);
-    const EvalDesc own = make_desc(ty, tz, y0, z0, A, TY);
+    EvalDesc own = make_desc(ty, tz, y0, z0, A, TY);
     EvalDesc halo = own;
     if (ty == 0) halo = make_desc(-1, tz, y0, z0, A, TY);
     else if (ty == 1) halo = make_desc(TY, tz, y0, z0, A, TY);
     else if (ty == 2 && tz < 2 * TY) halo = make_desc(tz < TY ? tz : tz - TY, tz < TY ? -1 : TZ, y0, z0, A, TY);
+    own.walls = __any(own.mask != 0);
+    halo.walls = __any(halo.mask != 0);
     const bool zgrav = A.gravity != 0;
     const T hz = zgrav ? grav_h<T>(zc, nz, C.ds) : T(0);
     // periodic: the reference copies the lower z face from face (n-1, n)
@@ -782,7 +827,7 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
                 const T u = uold[f];
                 T acc;
                 if constexpr (!FAST) acc = ((((fxu[f] - fxlo[f]) + fyu[f]) - fyl[f]) + fzu[f]) - fzl[f];
-                else acc = xlo_out ? fyu[f] : fxu[f] + fyu[f];   // fyu: y/z sum minus the lower x face
+                else acc = dfma<T>(xlo_out ? T(0) : T(1), fxu[f], fyu[f]);   // fyu: y/z sum minus the lower x face
                 un[f] = FAST ? dfma<T>(-C.dtds, acc, u) : u - C.dtds * acc;
                 at(Vout.f[f], uidx) = un[f];
             }
@@ -792,10 +837,12 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
                 double wd[5];
                 for (int f = 0; f < 5; ++f) wd[f] = (double)w[f];
                 const double den = wave_den<double, false>(wd, (double)C.gamma);
-                if (den != den) nanf = 1; else dmaxden = fmax(dmaxden, den);
+                nanf |= den != den;
+                dmaxden = fmax(dmaxden, den);   // NaN-ignoring
             } else {
                 const T den = wave_den<T, FAST>(w, C.gamma);
-                if (den != den) nanf = 1; else maxden = dmax<T>(maxden, den);
+                nanf |= den != den;
+                maxden = dmax<T>(maxden, den);  // NaN-ignoring
             }
         }
         if (full) {
@@ -807,7 +854,7 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
                     fyu[f] = fyu_n[f]; fyl[f] = fyl_n[f]; fzu[f] = fzu_n[f]; fzl[f] = fzl_n[f];
                 } else {
                     const T yz = ((fyu_n[f] - fyl_n[f]) + fzu_n[f]) - fzl_n[f];
-                    fyu[f] = xlo_next ? yz : yz - fxu[f];
+                    fyu[f] = dfma<T>(xlo_next ? T(0) : T(-1), fxu[f], yz);
                 }
             }
         }

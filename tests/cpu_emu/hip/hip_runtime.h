@@ -65,6 +65,21 @@ inline int exch_dpp(int old, int v, int dir)
 }  // namespace emu
 
 inline void __syncthreads() { emu::sync(); }
+// wave vote; every thread of the block must reach it (as every call site in
+// the kernels does)
+inline int __any(int p)
+{
+    const int tid = (int)threadIdx.x, wave = tid & ~63;
+    emu::sync();
+    emu::xbuf[tid] = p ? 1.0 : 0.0;
+    emu::sync();
+    int r = 0;
+    for (int l = 0; l < 64 && wave + l < (int)blockDim.x; ++l) r |= emu::xbuf[wave + l] != 0.0;
+    emu::sync();
+    return r;
+}
+// only called with wave-uniform values in the kernels
+inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
 template <typename T> inline T __shfl_down(T v, unsigned d) { return emu::exch<T>(v, (int)d, 0); }
 template <typename T> inline T __shfl_up(T v, unsigned d) { return emu::exch<T>(v, (int)d, 1); }
 template <typename T> inline T __shfl_xor(T v, int m) { return emu::exch<T>(v, m, 2); }
