@@ -49,7 +49,7 @@ def traffic_from_profile(name):
         return None
 
 
-def cpu_baseline(n_sample=128, steps=3):
+def cpu_baseline(n_sample=256, steps=12):
     """Reference CPU path: the reference's own Fortran physics module
     (oracle/_ref harness, real(4) build, gamma 1.4 / CFL 0.4) on a KHI sample
     of n_sample^3 cells, all host cores; falls back to the C oracle (a port)
@@ -78,7 +78,7 @@ def cpu_baseline(n_sample=128, steps=3):
             "sample": f"KHI {n_sample}^3, {steps} steps, {what}, {wall:.2f} s"}
 
 
-def cpu_baseline_ensemble(members=32, steps=10, seed=0):
+def cpu_baseline_ensemble(members=64, steps=20, seed=0):
     """CPU path for the ensemble workload: the C oracle (a port of
     physics.rs with the minmod slopes the ensemble uses; the reference has no
     limited scheme to time) on the first `members` members of the seeded
@@ -330,7 +330,8 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32_bulk", "fp64"])
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=256, help="edge of the KHI sample the CPU path runs")
+    ap.add_argument("--cpu-steps", type=int, default=12, help="steps of the CPU sample (~10 s on 16 cores)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (scale for rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -361,7 +362,8 @@ def main():
         r = bench_khi(args)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline_ensemble() if args.workload == "ensemble" else cpu_baseline(args.cpu_sample)
+        cpu = cpu_baseline_ensemble() if args.workload == "ensemble" else cpu_baseline(args.cpu_sample,
+                                                                                      args.cpu_steps)
     out = {
         "metric": r.get("metric", "Gcell-updates/s + % HBM peak"),
         "value": round(r["value"], 3),
