@@ -140,7 +140,7 @@ int dg_set_heat(dg_ctx *c, double t_out_kelvin, double v_out, double rs)
     c->heat_rs = rs;
     for (int b = 0; b < 2; ++b)
         if (c->graph[b]) {   // the captured steps hold the constants
-            hipGraphExecDestroy(c->graph[b]);
+            (void)hipGraphExecDestroy(c->graph[b]);
             c->graph[b] = nullptr;
         }
     return DG_OK;
@@ -196,7 +196,7 @@ int set_tiling(dg_ctx *c, int tile_y, int chunk_x)
     c->chunk_x = lx;
     for (int b = 0; b < 2; ++b)
         if (c->graph[b]) {
-            hipGraphExecDestroy(c->graph[b]);
+            (void)hipGraphExecDestroy(c->graph[b]);
             c->graph[b] = nullptr;
         }
     return DG_OK;
@@ -209,7 +209,7 @@ int reserve_partials(dg_ctx *c)
     if (c->partials && c->partials_cap >= need) return DG_OK;
     if (c->partials) {
         if (hipStreamSynchronize(c->stream) != hipSuccess) return DG_ERR_DEVICE;
-        hipFree(c->partials);
+        (void)hipFree(c->partials);
         c->partials = nullptr;
         c->partials_cap = 0;
     }
@@ -346,18 +346,18 @@ extern "C" {
 void dg_destroy(dg_ctx *c)
 {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int b = 0; b < 2; ++b)
-        if (c->graph[b]) hipGraphExecDestroy(c->graph[b]);
+        if (c->graph[b]) (void)hipGraphExecDestroy(c->graph[b]);
     for (int b = 0; b < 2; ++b)
         for (int f = 0; f < 5; ++f)
-            if (c->buf[b][f]) hipFree(c->buf[b][f]);
-    if (c->partials) hipFree(c->partials);
-    if (c->sc) hipFree(c->sc);
-    if (c->scratch) hipFree(c->scratch);
-    if (c->sc_host) hipHostFree(c->sc_host);
-    if (c->stage) hipHostFree(c->stage);
-    if (c->stream && c->owns_stream) hipStreamDestroy(c->stream);
+            if (c->buf[b][f]) (void)hipFree(c->buf[b][f]);
+    if (c->partials) (void)hipFree(c->partials);
+    if (c->sc) (void)hipFree(c->sc);
+    if (c->scratch) (void)hipFree(c->scratch);
+    if (c->sc_host) (void)hipHostFree(c->sc_host);
+    if (c->stage) (void)hipHostFree(c->stage);
+    if (c->stream && c->owns_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -463,10 +463,10 @@ static int ensure_graph(dg_ctx *c)
     for (int s = 0; s < kGraphSteps && rc == DG_OK; ++s) rc = dg::launch_one_step(c);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->cur = cur0;  // capture launched nothing
-    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
     if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
     e = hipGraphInstantiate(&c->graph[cur0], g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) { c->graph[cur0] = nullptr; return hip_fail(c, e, "hipGraphInstantiate"); }
     c->graph_steps = kGraphSteps;
     return DG_OK;
@@ -575,12 +575,12 @@ int dg_time_steps(dg_ctx *c, int64_t nsteps, double *ms_total, double *ms_step_k
     double tk = 0, tr = 0;
     float ms = 0;
     for (int64_t s = 0; s < nsteps && rc == DG_OK; ++s) {
-        hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]);
+        (void)hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]);
         tk += ms;
-        hipEventElapsedTime(&ms, ev[2 * s + 1], ev[2 * s + 2]);
+        (void)hipEventElapsedTime(&ms, ev[2 * s + 1], ev[2 * s + 2]);
         tr += ms;
     }
-    hipEventElapsedTime(&ms, ev[0], ev[2 * nsteps]);
+    (void)hipEventElapsedTime(&ms, ev[0], ev[2 * nsteps]);
     if (rc) return rc;
     if (ms_total) *ms_total = ms;
     if (ms_step_kernel) *ms_step_kernel = tk / nsteps;

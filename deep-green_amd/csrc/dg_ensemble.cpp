@@ -132,10 +132,10 @@ int capture(dg_ensemble *e, int64_t nsteps, hipGraphExec_t *out)
     int rc = DG_OK;
     for (int64_t s = 0; s < nsteps && rc == DG_OK; ++s) rc = launch_one(e, e->parity ^ (int)(s & 1));
     hipError_t he = hipStreamEndCapture(e->stream, &g);
-    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
     if (he != hipSuccess) return efail(e, DG_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(he));
     he = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (he != hipSuccess) {
         *out = nullptr;
         return efail(e, DG_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(he));
@@ -164,10 +164,10 @@ void drop_graphs(dg_ensemble *e)
 {
     for (int b = 0; b < 2; ++b)
         if (e->graph[b]) {
-            hipGraphExecDestroy(e->graph[b]);
+            (void)hipGraphExecDestroy(e->graph[b]);
             e->graph[b] = nullptr;
         }
-    for (auto &r : e->runs) hipGraphExecDestroy(r.exec);
+    for (auto &r : e->runs) (void)hipGraphExecDestroy(r.exec);
     e->runs.clear();
 }
 
@@ -326,16 +326,16 @@ int dg_ensemble_create(const dg_config *cfgs, int count, int grid, dg_ensemble *
 void dg_ensemble_destroy(dg_ensemble *e)
 {
     if (!e) return;
-    if (e->stream) hipStreamSynchronize(e->stream);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graphs(e);
     for (dg_ctx *c : e->inst) dg_destroy(c);
-    if (e->d_inst) hipFree(e->d_inst);
-    if (e->d_items) hipFree(e->d_items);
-    if (e->partials) hipFree(e->partials);
-    if (e->d_count) hipFree(e->d_count);
-    if (e->d_next) hipFree(e->d_next);
-    if (e->h_count) hipHostFree(e->h_count);
-    if (e->stream) hipStreamDestroy(e->stream);
+    if (e->d_inst) (void)hipFree(e->d_inst);
+    if (e->d_items) (void)hipFree(e->d_items);
+    if (e->partials) (void)hipFree(e->partials);
+    if (e->d_count) (void)hipFree(e->d_count);
+    if (e->d_next) (void)hipFree(e->d_next);
+    if (e->h_count) (void)hipHostFree(e->h_count);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
 
@@ -452,9 +452,9 @@ int dg_ensemble_time_steps(dg_ensemble *e, int64_t nsteps, double *ms_step_kerne
     double tk = 0, tf = 0;
     float ms = 0;
     for (int64_t s = 0; s < nsteps && rc == DG_OK; ++s) {
-        hipEventElapsedTime(&ms, ev[3 * s], ev[3 * s + 1]);
+        (void)hipEventElapsedTime(&ms, ev[3 * s], ev[3 * s + 1]);
         tk += ms;
-        hipEventElapsedTime(&ms, ev[3 * s + 1], ev[3 * s + 2]);
+        (void)hipEventElapsedTime(&ms, ev[3 * s + 1], ev[3 * s + 2]);
         tf += ms;
     }
     if (rc) return rc;

@@ -59,7 +59,7 @@ struct EventSet {
     ~EventSet()
     {
         for (auto &e : ev)
-            if (e) hipEventDestroy(e);
+            if (e) (void)hipEventDestroy(e);
     }
 };
 

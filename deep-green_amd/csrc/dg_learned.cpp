@@ -109,8 +109,8 @@ void *ws(dg_cnn *m, int k, size_t n)
     Buf &b = m->ws[k];
     if (b.n >= n && b.p) return b.p;
     if (b.p) {
-        hipStreamSynchronize(m->stream);
-        hipFree(b.p);
+        (void)hipStreamSynchronize(m->stream);
+        (void)hipFree(b.p);
     }
     b.p = nullptr;
     b.n = 0;
@@ -219,8 +219,8 @@ int upload_tables(dg_cnn *m)
 {
     const size_t n = m->sU.size();
     if (n > m->d_cap) {
-        if (m->d_sU) hipFree(m->d_sU);
-        if (m->d_sR) hipFree(m->d_sR);
+        if (m->d_sU) (void)hipFree(m->d_sU);
+        if (m->d_sR) (void)hipFree(m->d_sR);
         m->d_sU = m->d_sR = nullptr;
         m->d_cap = 0;
         const size_t cap = std::max<size_t>(n * 2, 64);
@@ -281,7 +281,7 @@ void drop_graphs(dg_cnn *m)
 {
     for (int b = 0; b < 2; ++b)
         if (m->graph[b]) {
-            hipGraphExecDestroy(m->graph[b]);
+            (void)hipGraphExecDestroy(m->graph[b]);
             m->graph[b] = nullptr;
         }
 }
@@ -308,7 +308,7 @@ int prepare_infer(dg_cnn *m, dg_ctx *c)
         m->graph_n3 = c->n3;
     }
     if (m->flags_n < c->n3) {
-        if (m->flags) hipFree(m->flags);
+        if (m->flags) (void)hipFree(m->flags);
         m->flags = nullptr;
         m->flags_n = 0;
         MHIP(m, hipMalloc((void **)&m->flags, c->n3));
@@ -473,28 +473,28 @@ int dg_cnn_create(int device, uint64_t seed, dg_cnn **out)
 void dg_cnn_destroy(dg_cnn *m)
 {
     if (!m) return;
-    if (m->stream) hipStreamSynchronize(m->stream);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
     for (int b = 0; b < 2; ++b)
-        if (m->graph[b]) hipGraphExecDestroy(m->graph[b]);
+        if (m->graph[b]) (void)hipGraphExecDestroy(m->graph[b]);
     float *fs[] = {m->params, m->grads, m->adam_m, m->adam_v, m->dw_part, m->db_part, m->tmp,
                    m->bias_pad[0], m->bias_pad[1], m->bias_pad[2]};
     for (float *f : fs)
-        if (f) hipFree(f);
+        if (f) (void)hipFree(f);
     for (int l = 0; l < 3; ++l) {
-        if (m->Wp[l]) hipFree(m->Wp[l]);
-        if (m->Wt[l]) hipFree(m->Wt[l]);
+        if (m->Wp[l]) (void)hipFree(m->Wp[l]);
+        if (m->Wt[l]) (void)hipFree(m->Wt[l]);
     }
-    for (float *f : m->sU) hipFree(f);
-    for (float *f : m->sR) hipFree(f);
-    if (m->d_sU) hipFree(m->d_sU);
-    if (m->d_sR) hipFree(m->d_sR);
+    for (float *f : m->sU) (void)hipFree(f);
+    for (float *f : m->sR) (void)hipFree(f);
+    if (m->d_sU) (void)hipFree(m->d_sU);
+    if (m->d_sR) (void)hipFree(m->d_sR);
     for (Buf &b : m->ws)
-        if (b.p) hipFree(b.p);
-    if (m->loss_part) hipFree(m->loss_part);
-    if (m->d_ptrs) hipFree(m->d_ptrs);
-    if (m->d_box0) hipFree(m->d_box0);
-    if (m->flags) hipFree(m->flags);
-    if (m->stream) hipStreamDestroy(m->stream);
+        if (b.p) (void)hipFree(b.p);
+    if (m->loss_part) (void)hipFree(m->loss_part);
+    if (m->d_ptrs) (void)hipFree(m->d_ptrs);
+    if (m->d_box0) (void)hipFree(m->d_box0);
+    if (m->flags) (void)hipFree(m->flags);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
 
@@ -656,8 +656,8 @@ int dg_cnn_add_sample(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse)
     std::vector<float *> u(5), r(5);
     for (int q = 0; q < 5; ++q) {
         if (hipMalloc((void **)&u[q], n3 * 4) != hipSuccess || hipMalloc((void **)&r[q], n3 * 4) != hipSuccess) {
-            for (float *f : u) if (f) hipFree(f);
-            for (float *f : r) if (f) hipFree(f);
+            for (float *f : u) if (f) (void)hipFree(f);
+            for (float *f : r) if (f) (void)hipFree(f);
             return mfail(m, DG_ERR_ALLOC, "sample storage");
         }
         MHIP(m, hipMemcpyAsync(u[q], coarse->buf[coarse->cur][q], n3 * 4, hipMemcpyDeviceToDevice, coarse->stream));
@@ -677,7 +677,7 @@ int dg_cnn_add_sample(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse)
         if (dg::launch_one_step(fine)) return mfail(m, DG_ERR_DEVICE, fine->err);
     }
     if (m->tmp_cells < n3) {
-        if (m->tmp) hipFree(m->tmp);
+        if (m->tmp) (void)hipFree(m->tmp);
         m->tmp = nullptr;
         MHIP(m, hipMalloc((void **)&m->tmp, 5 * n3 * 4));
         m->tmp_cells = n3;
@@ -824,10 +824,10 @@ int dg_learned_run(dg_ctx *c, dg_cnn *m, int64_t max_steps, double t_end, double
                 for (int k = 0; k < kGraphSteps && rc == DG_OK; ++k) rc = learned_launch(m, c);
                 hipError_t e = hipStreamEndCapture(m->stream, &g);
                 c->cur = cur0;
-                if (rc) { if (g) hipGraphDestroy(g); return rc; }
+                if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
                 if (e != hipSuccess) return mfail(m, DG_ERR_DEVICE, "hipStreamEndCapture");
                 e = hipGraphInstantiate(&m->graph[cur0], g, nullptr, nullptr, 0);
-                hipGraphDestroy(g);
+                (void)hipGraphDestroy(g);
                 if (e != hipSuccess) { m->graph[cur0] = nullptr; return mfail(m, DG_ERR_DEVICE, "hipGraphInstantiate"); }
             }
             MHIP(m, hipGraphLaunch(m->graph[c->cur], m->stream));
@@ -897,10 +897,10 @@ int dg_learned_time(dg_cnn *m, dg_ctx *c, int nrep, double ms[5])
         MHIP(m, hipEventSynchronize(ev[4]));
         float t;
         for (int k = 0; k < 4; ++k) {
-            hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+            (void)hipEventElapsedTime(&t, ev[k], ev[k + 1]);
             acc[k] += t;
         }
-        hipEventElapsedTime(&t, ev[0], ev[4]);
+        (void)hipEventElapsedTime(&t, ev[0], ev[4]);
         acc[4] += t;
     }
     for (int k = 0; k < 5; ++k) ms[k] = acc[k] / nrep;
