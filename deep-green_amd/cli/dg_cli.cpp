@@ -199,6 +199,79 @@ void init_trbs(Grid &g)
     bubble(g, c3, 1.5f * 15.0f, 1.5f * 10.0f);
 }
 
+// Benchmark configurations (gamma 1.4, CFL 0.4, no gravity, unit cube,
+// cell centres x_i = (i - 2 + 1/2) ds): the Sod tube, the blast wave and the
+// Kelvin-Helmholtz layer the benchmark is quoted on.
+float centre(int i, float ds) { return ((float)i - 2.0f + 0.5f) * ds; }
+
+void init_sod(Grid &g)
+{
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                const bool left = centre(i, g.ds) < 0.5f;
+                g.at(0, i, j, k) = left ? 1.0f : 0.125f;
+                g.at(4, i, j, k) = left ? 1.0f : 0.1f;
+            }
+}
+
+void init_blast(Grid &g)
+{
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                const float x = centre(i, g.ds) - 0.5f, y = centre(j, g.ds) - 0.5f, z = centre(k, g.ds) - 0.5f;
+                g.at(0, i, j, k) = 1.0f;
+                g.at(4, i, j, k) = x * x + y * y + z * z < 0.1f * 0.1f ? 10.0f : 0.1f;
+            }
+}
+
+// v perturbation: 0.01 sin(4 pi x) plus uniform noise in [-1e-3, 1e-3] from a
+// seeded splitmix64 stream (the same distribution as the Python set-up, not
+// the same numbers)
+std::function<void(Grid &)> init_khi_layer(uint64_t seed)
+{
+    return [seed](Grid &g) {
+        uint64_t st = seed;
+        auto uni = [&st]() {
+            uint64_t z = (st += 0x9E3779B97F4A7C15ULL);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+            z ^= z >> 31;
+            return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+        };
+        for (int i = 0; i < g.nx; ++i) {
+            const float sx = 0.01f * std::sin(4.0f * kPi * centre(i, g.ds));
+            for (int j = 0; j < g.ny; ++j) {
+                const bool inner = std::fabs(centre(j, g.ds) - 0.5f) < 0.25f;
+                for (int k = 0; k < g.nz; ++k) {
+                    g.at(0, i, j, k) = inner ? 2.0f : 1.0f;
+                    g.at(1, i, j, k) = inner ? 0.5f : -0.5f;
+                    g.at(2, i, j, k) = sx + (float)(2e-3 * uni() - 1e-3);
+                    g.at(4, i, j, k) = 2.5f;
+                }
+            }
+        }
+    };
+}
+
+bool find_config(const std::string &name, uint64_t seed, Demo &d, std::string &precision)
+{
+    if (name == "sod64") {
+        d = {{64, 64, 64}, 1.0f / 64, {O, O, O, O, O, O}, false, init_sod};
+        precision = "fp64";
+    } else if (name == "blast256") {
+        d = {{256, 256, 256}, 1.0f / 256, {P, P, P, P, P, P}, false, init_blast};
+        precision = "fp64";
+    } else if (name == "khi512") {
+        d = {{512, 512, 512}, 1.0f / 512, {P, P, P, P, P, P}, false, init_khi_layer(seed)};
+        precision = "fp32";
+    } else {
+        return false;
+    }
+    return true;
+}
+
 // set_up_*_demo (init.rs:8-321) and the dispatch of main.rs:79-108
 bool find_demo(const std::string &name, Demo &d)
 {
@@ -267,6 +340,9 @@ void usage()
                  "usage: dg_cli --demo NAME --tout T --total-time T --max-steps N\n"
                  "              [--precision fp32|fp64|fp32_bulk] [--math exact|fast]\n"
                  "              [--limiter none|minmod|vanleer] [--out DIR] [--no-dump] [--quiet]\n"
+                 "       dg_cli --config sod64|blast256|khi512 --steps N [--seed S] [options above]\n"
+                 "              (gamma 1.4, CFL 0.4, no gravity; dumps only with --out: after the\n"
+                 "              first step, then every --tout; default precision fp64, fp64, fp32)\n"
                  "demos: KELVIN_HELMHOLTZ THERMAL_RISING_BUBBLE THERMAL_RISING_BUBBLES EXPLOSION RELAXATION\n"
                  "       SHOCK_TUBE RAYLEIGH_TAYLOR RAYLEIGH_BERNARD\n");
 }
@@ -275,10 +351,11 @@ void usage()
 
 int main(int argc, char **argv)
 {
-    std::string demo, out = ".", precision = "fp32", math = "exact", limiter = "none";
+    std::string demo, config, out = ".", precision, math = "exact", limiter = "none";
     float tout = -1, total_time = -1;
     long long max_steps = -1;
-    bool do_dump = true, quiet = false;
+    uint64_t seed = 1;
+    bool do_dump = true, quiet = false, out_given = false;
     for (int a = 1; a < argc; ++a) {
         const std::string k = argv[a];
         auto val = [&]() -> std::string {
@@ -292,21 +369,44 @@ int main(int argc, char **argv)
         else if (k == "--precision") precision = val();
         else if (k == "--math") math = val();
         else if (k == "--limiter") limiter = val();
-        else if (k == "--out") out = val();
+        else if (k == "--out") { out = val(); out_given = true; }
         else if (k == "--no-dump") do_dump = false;
         else if (k == "--quiet") quiet = true;
+        else if (k == "--config") config = val();
+        else if (k == "--steps") max_steps = std::strtoll(val().c_str(), nullptr, 10);
+        else if (k == "--seed") seed = std::strtoull(val().c_str(), nullptr, 10);
         else { usage(); return 2; }
     }
-    if (demo.empty() || tout < 0 || total_time < 0 || max_steps < 0) { usage(); return 2; }
     Demo d;
-    if (!find_demo(demo, d)) {
-        std::fprintf(stderr, "Demo not found! Available demos are:\n");
-        usage();
-        return 0;   // the reference exits 0 here too (main.rs:103-108)
+    std::string default_precision = "fp32";
+    if (!config.empty()) {
+        if (!demo.empty() || max_steps < 0) { usage(); return 2; }
+        if (!find_config(config, seed, d, default_precision)) {
+            std::fprintf(stderr, "Configuration not found!\n");
+            usage();
+            return 2;
+        }
+        demo = config;
+        if (total_time < 0) total_time = INFINITY;
+        if (tout < 0) tout = INFINITY;
+        if (!out_given) do_dump = false;   // a 512^3 dump is 11 GB: only when asked for
+    } else {
+        if (demo.empty() || tout < 0 || total_time < 0 || max_steps < 0) { usage(); return 2; }
+        if (!find_demo(demo, d)) {
+            std::fprintf(stderr, "Demo not found! Available demos are:\n");
+            usage();
+            return 0;   // the reference exits 0 here too (main.rs:103-108)
+        }
     }
+    if (precision.empty()) precision = default_precision;
     const auto t0 = std::chrono::steady_clock::now();
     dg_config cfg;
     dg_config_default(&cfg);   // GAMMA, CFL, G of globals.rs
+    if (!config.empty()) {
+        cfg.gamma = 1.4;
+        cfg.cfl = 0.4;
+        cfg.g = 0.0;
+    }
     for (int i = 0; i < 3; ++i) cfg.n[i] = d.n[i];
     for (int i = 0; i < 6; ++i) cfg.bc[i] = d.bc[i];
     cfg.ds = d.ds;

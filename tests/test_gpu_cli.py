@@ -58,3 +58,30 @@ def test_cli_unknown_demo_lists_demos(tmp_path):
     out = subprocess.run([CLI, "--demo", "NOPE", "--tout", "1", "--total-time", "1", "--max-steps", "1"],
                          capture_output=True, text=True, timeout=60)
     assert out.returncode == 0 and "KELVIN_HELMHOLTZ" in out.stderr
+
+
+def test_cli_benchmark_config_sod64_follows_oracle(tmp_path):
+    """--config sod64 (gamma 1.4, CFL 0.4, outflow, FP64): one library step
+    between two dumps equals the FP64 oracle's step bit for bit."""
+    out = run_cli(["--config", "sod64", "--steps", "2", "--tout", "0", "--out", str(tmp_path), "--quiet"],
+                  str(tmp_path))
+    assert "2 steps" in out
+    dumps = sorted(d for d in os.listdir(tmp_path) if d.startswith("state."))
+    assert dumps == ["state.00000", "state.00001"]
+    U0 = load(tmp_path / dumps[0], "conserved", FIELDS)
+    U1 = load(tmp_path / dumps[1], "conserved", FIELDS)
+    assert U0[0].shape == (68, 68, 68)
+    o = Oracle((64, 64, 64), 1.0 / np.float32(64), [1] * 6, precision="f64", gamma=1.4, cfl=0.4, g=0.0,
+               gravity=False, gravity_k_shift=0)
+    o.set_conserved(U0)
+    o.steps(1)
+    for q in range(5):
+        np.testing.assert_array_equal(np.array(o.interior_view(o.cons[q])), U1[q][2:-2, 2:-2, 2:-2])
+    # the tube: left state rho 1, right 0.125, far from the interface untouched after two steps
+    W1 = load(tmp_path / dumps[1], "primitives", ("rho", "vx", "vy", "vz", "pressure"))
+    assert W1[0][2, 10, 10] == 1.0 and abs(W1[0][-3, 10, 10] - 0.125) < 1e-7
+
+
+def test_cli_benchmark_config_without_out_writes_nothing(tmp_path):
+    out = run_cli(["--config", "blast256", "--steps", "1", "--precision", "fp32", "--quiet"], str(tmp_path))
+    assert "1 steps" in out and not os.listdir(tmp_path)
