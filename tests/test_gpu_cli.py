@@ -85,3 +85,10 @@ def test_cli_benchmark_config_sod64_follows_oracle(tmp_path):
 def test_cli_benchmark_config_without_out_writes_nothing(tmp_path):
     out = run_cli(["--config", "blast256", "--steps", "1", "--precision", "fp32", "--quiet"], str(tmp_path))
     assert "1 steps" in out and not os.listdir(tmp_path)
+
+
+def test_cli_benchmark_config_khi512_runs(tmp_path):
+    """The 512^3 benchmark configuration end to end from the command line
+    (fast math, as bench.py times it)."""
+    out = run_cli(["--config", "khi512", "--steps", "3", "--math", "fast", "--quiet"], str(tmp_path))
+    assert "Running simulation khi512: 512x512x512 cells" in out and "3 steps" in out
