@@ -302,6 +302,14 @@ int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out)
     a.limiter = cfg->limiter;
     a.gravity = cfg->gravity;
     a.ds = cfg->ds;
+    {
+        // a power of two whose cube and halves stay normal in float: dividing by
+        // ds, 2 ds or ds^3 is then exactly a multiplication by the reciprocal
+        // (both round the same real number), which the exact kernels use
+        int ex = 0;
+        const double mant = std::frexp(cfg->ds, &ex);
+        a.pow2 = (mant == 0.5 && ex > -40 && ex < 40) ? 1 : 0;
+    }
     a.gamma = cfg->gamma;
     a.g = cfg->g;
     c->nparts_reduce = dg::reduce_blocks(a);

@@ -26,6 +26,8 @@ struct StepArgs {
     int nbx, nby, nbz; // workgroups per axis
     int nblocks;
     int swizzle;       // 1: spread tiles so each XCD works on a contiguous range
+    int pow2;          // 1: ds is a power of two, so x / ds, x / (2 ds) and x / ds^3
+                       //    equal the products with the exact reciprocals bit for bit
     double ds, gamma, g;
 };
 

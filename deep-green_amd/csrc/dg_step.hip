@@ -149,14 +149,15 @@ __device__ __forceinline__ bool wall_ghost(int p, int o, int n, int lo, int hi)
 }
 
 // calc_primitives (physics.rs:109-141) for one cell.
+// pow2: V is a power of two, so u / V == u * invV exactly.
 template <typename T, bool FAST>
-__device__ __forceinline__ void to_prim(const T u[5], T V, T invV, T gm1, T w[5])
+__device__ __forceinline__ void to_prim(const T u[5], T V, T invV, T gm1, T w[5], bool pow2 = false)
 {
     if constexpr (!FAST) {
-        const T r = u[0] / V;
+        const T r = pow2 ? u[0] * invV : u[0] / V;
         const T ux = u[1] / (r * V), uy = u[2] / (r * V), uz = u[3] / (r * V);
         w[0] = r; w[1] = ux; w[2] = uy; w[3] = uz;
-        w[4] = (u[4] / V - T(0.5) * r * (ux * ux + uy * uy + uz * uz)) * gm1;
+        w[4] = ((pow2 ? u[4] * invV : u[4] / V) - T(0.5) * r * (ux * ux + uy * uy + uz * uz)) * gm1;
     } else {
         const T r = u[0] * invV;
         const T im = drcp<T>(u[0]);
@@ -270,7 +271,8 @@ __device__ __forceinline__ int rot_field(int axis, int slot)
 
 template <typename T>
 struct Consts {
-    T ds, gamma, gm1, igm1, V, invV, two_ds, inv_two_ds, half_ds, half_dt, dtds, G;
+    T ds, gamma, gm1, igm1, V, invV, two_ds, inv_two_ds, inv_ds, half_ds, half_dt, dtds, G;
+    bool pow2;   // ds a power of two: divisions by ds, 2 ds, ds^3 as exact products
 };
 
 // Two-element vector for 8-byte (float) / 16-byte (double) LDS accesses.
@@ -305,10 +307,10 @@ template <typename T, bool FAST, int LIM>
 __device__ __forceinline__ T slope(T m, T c, T p, const Consts<T> &C)
 {
     if constexpr (LIM == LIM_NONE) {
-        if constexpr (!FAST) return (p - m) / C.two_ds;
+        if constexpr (!FAST) return C.pow2 ? (p - m) * C.inv_two_ds : (p - m) / C.two_ds;
         else return p - m;
     } else {
-        if constexpr (!FAST) return limited<T>(c - m, p - c, LIM) / C.ds;
+        if constexpr (!FAST) return C.pow2 ? limited<T>(c - m, p - c, LIM) * C.inv_ds : limited<T>(c - m, p - c, LIM) / C.ds;
         else return T(2) * limited<T>(c - m, p - c, LIM);
     }
 }
@@ -539,7 +541,7 @@ __device__ __forceinline__ void finish_plane(StepLDS<T, TY> &L, int slot, const 
         const bool flips = __any(Q.xflip || (P.lds[q] >= 0 && (P.lds[q] & (3 << 13)) != 0));
         if (P.lds[q] < 0) continue;
         T w[6];
-        to_prim<T, FAST>(Q.u[q], C.V, C.invV, C.gm1, w);
+        to_prim<T, FAST>(Q.u[q], C.V, C.invV, C.gm1, w, C.pow2);
         // fast math: 1/rho = V / U_rho for the Hancock step; exact math pads
         w[5] = FAST ? C.V * drcp<T>(Q.u[q][0]) : T(0);
         if (flips) {
@@ -626,6 +628,8 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
     C.invV = T(1) / C.V;
     C.two_ds = T(2.0) * C.ds;
     C.inv_two_ds = T(1) / C.two_ds;
+    C.inv_ds = T(1) / C.ds;
+    C.pow2 = A.pow2 != 0;
     C.half_ds = FAST ? T(0.25) : C.ds / T(2.);   // fast: slopes are scaled by 2 ds
     C.G = T(A.g);
     const T dt = T(sc->dt);
@@ -832,7 +836,7 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
                 at(Vout.f[f], uidx) = un[f];
             }
             T w[5];
-            to_prim<T, FAST>(un, C.V, C.invV, C.gm1, w);
+            to_prim<T, FAST>(un, C.V, C.invV, C.gm1, w, C.pow2);
             if constexpr (BULK) {
                 double wd[5];
                 for (int f = 0; f < 5; ++f) wd[f] = (double)w[f];
