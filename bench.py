@@ -266,19 +266,21 @@ def bench_blast(args):
     """Config: 3D blast wave, 256^3 periodic unit cube, gamma 1.4, CFL 0.4,
     background rho 1, p 0.1, p 10 inside r < 0.1 about the centre, 500
     steps, minmod slopes (the reference's unlimited slopes turn the blast
-    non-finite in step 2), FP64 and FP32-bulk from the same initial state;
-    reports the FP32-bulk error against FP64 (relative L1 and Linf per
-    conserved field) and both throughputs."""
+    non-finite in step 2), FP64 and FP32-bulk (exact math, and fast math as a
+    second line) from the same initial state; reports the FP32-bulk errors
+    against FP64 (relative L1 and Linf per conserved field) and the
+    throughputs."""
     import deepgreen as dg
     from deepgreen.setups import blast
 
     n = args.n_blast
     out = {}
     U = {}
-    for prec in ("fp64", "fp32_bulk"):
+    for key, prec, math_ in (("fp64", "fp64", "exact"), ("fp32_bulk", "fp32_bulk", "exact"),
+                             ("fp32_bulk_fast", "fp32_bulk", "fast")):
         w, ds = blast(n, np.float64 if prec == "fp64" else np.float32)
         sim = dg.Simulation(n, n, n, ds, [dg.PERIODIC] * 6, gamma=1.4, cfl=0.4, g=0.0, precision=prec,
-                            math="exact", limiter="minmod")
+                            math=math_, limiter="minmod")
         s = dg.Solver(sim)
         s.set_primitives(w)
         del w
@@ -287,25 +289,31 @@ def bench_blast(args):
         t, steps = s.run(args.steps)
         s.next_dt()
         el = time.perf_counter() - t0
-        out[prec] = {"gcells": n ** 3 * args.steps / el / 1e9, "t": t, "steps": steps}
-        U[prec] = np.stack([np.asarray(a[2:-2, 2:-2, 2:-2], np.float64) for a in s.get_state()])
+        out[key] = {"gcells": n ** 3 * args.steps / el / 1e9, "t": t, "steps": steps}
+        U[key] = np.stack([np.asarray(a[2:-2, 2:-2, 2:-2], np.float64) for a in s.get_state()])
         s.close()
-        print(f"[bench] blast {n}^3 {prec}: {args.steps} steps in {el:.3f} s, t = {t:.6f}", file=sys.stderr,
+        print(f"[bench] blast {n}^3 {key}: {args.steps} steps in {el:.3f} s, t = {t:.6f}", file=sys.stderr,
               flush=True)
     names = ("mass", "momentum_x", "momentum_y", "momentum_z", "energy")
-    err = {}
-    for q, nm in enumerate(names):
-        d = np.abs(U["fp32_bulk"][q] - U["fp64"][q])
-        ref_l1 = np.abs(U["fp64"][q]).mean()
-        ref_max = np.abs(U["fp64"][q]).max()
-        err[nm] = {"rel_l1": float(d.mean() / ref_l1), "rel_linf": float(d.max() / ref_max)}
+    errs = {}
+    for key in ("fp32_bulk", "fp32_bulk_fast"):
+        err = {}
+        for q, nm in enumerate(names):
+            d = np.abs(U[key][q] - U["fp64"][q])
+            ref_l1 = np.abs(U["fp64"][q]).mean()
+            ref_max = np.abs(U["fp64"][q]).max()
+            err[nm] = {"rel_l1": float(d.mean() / ref_l1), "rel_linf": float(d.max() / ref_max)}
+        errs[key] = err
     return {
         "s_per_step": 1.0 / (out["fp32_bulk"]["gcells"] * 1e9 / n ** 3), "value": out["fp32_bulk"]["gcells"],
         "dtype": "f32",
         "config": {"workload": f"blast{n}_fp64_vs_fp32bulk", "cells": n ** 3, "limiter": "minmod",
                    "fp64_gcells": round(out["fp64"]["gcells"], 3), "fp32_bulk_gcells": round(out["fp32_bulk"]["gcells"], 3),
+                   "fp32_bulk_fast_gcells": round(out["fp32_bulk_fast"]["gcells"], 3),
                    "t_fp64": out["fp64"]["t"], "t_fp32_bulk": out["fp32_bulk"]["t"],
-                   "fp32_bulk_error_vs_fp64": err},
+                   "t_fp32_bulk_fast": out["fp32_bulk_fast"]["t"],
+                   "fp32_bulk_error_vs_fp64": errs["fp32_bulk"],
+                   "fp32_bulk_fast_error_vs_fp64": errs["fp32_bulk_fast"]},
         "roofline": None,
     }
 
