@@ -973,7 +973,7 @@ reduce_kernel(const StepArgs A, const Fields<const T> U, double *__restrict__ pa
 
 // dt from the partials (calc_timestep, physics.rs:177-196: CFL * min over
 // cells of ds / den == CFL * ds / max den), conservation sums, and the run
-// clock (main.rs:296-298).  One workgroup of NT threads; fixed reduction
+// clock (main.rs:64-66).  One workgroup of NT threads; fixed reduction
 // order.  mode 0: dt and sums of a fresh state; 1: after a step (advance the
 // clock, new dt); 2: sums only.
 template <typename T, bool BULK, int NT>

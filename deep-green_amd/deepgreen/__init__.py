@@ -3,7 +3,7 @@
 Mirrors the reference Rust host's interface (src/rs/src/grid.rs, physics.rs,
 main.rs): a `Simulation` describes the grid (sizes without ghosts, ds, six
 boundary types, gravity) and a `Solver` owns the device state and offers
-`compute_step` (physics.rs:595) and `compute` (main.rs:270-300).  All work is
+`compute_step` (physics.rs:595) and `compute` (main.rs:38-68).  All work is
 done by the HIP library libdeepgreen.so through its C ABI
 (include/deepgreen.h); there is no CPU fallback: importing raises if the
 library is missing.

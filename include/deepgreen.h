@@ -6,7 +6,7 @@
  * set-up and its time loop, and replaces
  *     physics::compute_step(info, grid, &mut dt)      src/rs/src/physics.rs:595
  * and the loop around it
- *     compute(info, grid, total_time, max_steps, tout) src/rs/src/main.rs:270-300
+ *     compute(info, grid, total_time, max_steps, tout) src/rs/src/main.rs:38-68
  * by the calls below.  Plain C types only; every call returns 0 or a negative
  * dg_status and never aborts.  A dg_ctx is used by one host thread at a time.
  *
@@ -106,7 +106,7 @@ int dg_get_state(dg_ctx *ctx, void *const f[5], int kind);
  * current state (returned in *dt_used), advances the state by one step. */
 int dg_step(dg_ctx *ctx, double *dt_used);
 
-/* The loop of main.rs:283-299 without output: steps while
+/* The loop of main.rs:51-67 without output: steps while
  * !(t > t_end || steps >= max_steps), accumulating t += dt in the precision's
  * time type (float for DG_FP32, as the reference).  Runs from a captured HIP
  * graph with the dt kept on the device.  t and the step count continue from
@@ -159,7 +159,7 @@ int dg_device_info(int device, char *arch, int len, int *compute_units);
  * step strides over the (instance, tile) work items of every instance,
  * followed by one dt/clock kernel with a workgroup per instance; dg_ensemble_run
  * replays these from a captured HIP graph.  No reference counterpart: the
- * reference steps one grid per process (main.rs:283-299).  Every instance
+ * reference steps one grid per process (main.rs:51-67).  Every instance
  * computes exactly what dg_step / dg_run compute for the same dg_config and
  * state (same kernels, bit for bit), whatever the tiling. */
 typedef struct dg_ensemble dg_ensemble;

@@ -71,6 +71,8 @@ def lib():
                 ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
             getattr(_lib, f"dgo_steps_{sfx}").restype = ctypes.c_int64
             getattr(_lib, f"dgo_steps_{sfx}").argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+            getattr(_lib, f"dgo_limited_{sfx}").restype = rt
+            getattr(_lib, f"dgo_limited_{sfx}").argtypes = [rt, rt, ctypes.c_int]
             for nm in ("calc_primitives", "calc_conserved", "calc_gradients", "calc_fluxes",
                        "update_primitive_ghost_cells", "update_primitive_xtr_ghost_cells",
                        "update_gradients_ghost_cells", "update_fluxes_ghost_cells"):
@@ -247,3 +249,34 @@ def run_reference_fortran(w, ds, bcs, nsteps, precision="dp", params="cfg", nthr
     n3 = int(np.prod(shape))
     fields = [raw[nd + 1 + i * n3: nd + 1 + (i + 1) * n3].reshape(shape, order="F") for i in range(5)]
     return [np.ascontiguousarray(f) for f in fields], dts, wall
+
+
+def limited(a, b, limiter, precision="f64"):
+    """The oracle's limited slope of each (backward, forward) difference pair
+    (a[i], b[i]), in the given precision."""
+    L = lib()
+    fn = getattr(L, f"dgo_limited_{precision}")
+    dt = np.float32 if precision == "f32" else np.float64
+    lim = LIMITERS[limiter]
+    return np.array([fn(float(x), float(y), lim) for x, y in zip(np.asarray(a, dt), np.asarray(b, dt))], dtype=dt)
+
+
+def run_reference_limiters(a, b, precision="dp"):
+    """The reference Fortran module's minmod, maxmod and vanleerlimiter
+    (physics.f90:205-238) of each pair (a[i], b[i]), via oracle/_ref/dg_lim_*;
+    returns three float64 arrays."""
+    exe = os.path.join(REF_DIR, f"dg_lim_{precision}")
+    if not os.path.exists(exe):
+        raise FileNotFoundError("oracle/_ref binaries are not built")
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<i", a.size))
+            f.write(a.tobytes())
+            f.write(b.tobytes())
+        subprocess.run([exe, fin, fout], check=True)
+        raw = np.fromfile(fout, dtype=np.float64)
+    n = a.size
+    return raw[:n], raw[n:2 * n], raw[2 * n:3 * n]

@@ -275,6 +275,10 @@ static R FN(limited)(R a, R b, int limiter)
     return (R)2 * a * b / (a + b); /* van Leer */
 }
 
+/* The limited slope of one (backward, forward) difference pair, exported for
+ * the tests that pin it. */
+R FN(dgo_limited)(R a, R b, int limiter) { return FN(limited)(a, b, limiter); }
+
 /* calc_gradients (physics.rs:198-215) for all five primitives, on [1, n-1)^3.
  * With limiter != none the central difference is replaced by the limited
  * slope divided by ds. */
@@ -511,7 +515,7 @@ void FN(dgo_compute_step)(CTX *c, R *dt)
     FN(dgo_heat)(c, *dt);
 }
 
-/* compute (main.rs:270-300) without the HDF5 dump: calc_conserved, then step
+/* compute (main.rs:38-68) without the HDF5 dump: calc_conserved, then step
  * while !(tnow > total_time || step >= max_steps).  Time accumulates in R.
  * dt_hist (may be NULL) receives the dt of every step.  Returns the step
  * count; *t_out receives tnow. */

@@ -159,7 +159,7 @@ def test_conservation_sums_and_determinism():
 
 
 def test_compute_with_total_time_matches_reference_loop():
-    """main.rs:283-299: stop when tnow > total_time (f32 clock for FP32)."""
+    """main.rs:51-67: stop when tnow > total_time (f32 clock for FP32)."""
     from deepgreen import Simulation, Solver
     c = cases.build_case("smooth_mixed")
     prm = dict(cases.PARAMS["cfg"])
