@@ -37,16 +37,33 @@ sys.path.insert(0, os.path.join(ROOT, "deep-green_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
-def traffic_from_profile(name):
+def traffic_from_profile(name, kernel_ms):
     """HBM bytes per launch recorded in profiles/<name> (the line
-    'hbm_bytes_per_launch: <int>'), or None."""
+    'hbm_bytes_per_launch: <int>'), or None.  The summary also records the
+    kernel's mean duration in the profiled run ('kernel_ms: <float>'); a
+    summary whose duration differs from this run's by more than 10 % was
+    taken of another build of the kernel and is not used."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
-            m = re.search(r"^hbm_bytes_per_launch:\s*(\d+)", f.read(), re.M)
-        return int(m.group(1)) if m else None
+            txt = f.read()
     except OSError:
         return None
+    m = re.search(r"^hbm_bytes_per_launch:\s*(\d+)", txt, re.M)
+    t = re.search(r"^kernel_ms:\s*([0-9.eE+-]+)", txt, re.M)
+    if not m or not t or abs(float(t.group(1)) - kernel_ms) > 0.1 * kernel_ms:
+        return None
+    return int(m.group(1))
+
+
+GRAPH_STEPS = 16   # dg_run replays captured graphs of this many steps
+
+
+def warmup_steps(requested):
+    """Untimed warm-up length: at least one captured graph's worth of steps
+    and a multiple of it, so the graph the timed run replays exists (and is
+    instantiated) before the clock starts."""
+    return max(GRAPH_STEPS, -(-int(requested) // GRAPH_STEPS) * GRAPH_STEPS)
 
 
 def cpu_baseline(n_sample=256, steps=12):
@@ -117,7 +134,7 @@ def bench_khi(args):
     info = solver.kernel_info()
     print(f"[bench] KHI {n}^3 {args.precision}/{args.math} tile {info}", file=sys.stderr, flush=True)
 
-    solver.run(args.warmup)
+    solver.run(warmup_steps(args.warmup))
     # timed region: K steps from the captured graph, bracketed by device syncs
     solver.next_dt()
     t0 = time.perf_counter()
@@ -137,7 +154,7 @@ def bench_khi(args):
     finite = bool(np.isfinite(solver.conservation_sums()).all())
     solver.close()
     workload = f"khi{n}_{args.precision}_{args.math}"
-    traffic = traffic_from_profile(f"step_kernel_{workload}_hbm.md")
+    traffic = traffic_from_profile(f"step_kernel_{workload}_hbm.md", ms_step)
     return {
         "s_per_step": elapsed / args.steps, "value": gcells, "dtype": "f64" if args.precision == "fp64" else "f32",
         "config": {"workload": f"khi{n}_{args.precision}", "cells": cells, "periodic": True,
@@ -175,7 +192,9 @@ def bench_ensemble(args):
     print(f"[bench] ensemble of {len(sims)} ({dict(zip(sizes, counts))}), {info}, tiles {choice['tile_y']} "
           f"{choice['chunk_x']} waves {choice.get('waves', 4)} blocks/CU {choice['blocks_per_cu']}",
           file=sys.stderr, flush=True)
-    ens.run(args.warmup)
+    # the timed run replays one captured graph of exactly args.steps steps;
+    # the warm-up runs that graph once, so its capture is not timed
+    ens.run(args.steps)
     ens.clock(0)
     t0 = time.perf_counter()
     ens.run(args.steps)
@@ -189,7 +208,7 @@ def bench_ensemble(args):
     achieved = alg_bytes / (ms_step * 1e-3) / 1e9
     finite = all(bool(np.isfinite(ens.conservation_sums(i)).all()) for i in range(0, len(sims), 16))
     ens.close()
-    traffic = traffic_from_profile("ens_step_kernel_ensemble256_fp32_fast_hbm.md")
+    traffic = traffic_from_profile("ens_step_kernel_ensemble256_fp32_fast_hbm.md", ms_step)
     return {
         "s_per_step": elapsed / args.steps, "value": gcells, "dtype": "f32",
         "config": {"workload": f"ensemble{args.members}_fp32", "members": len(sims), "cells": cells,
@@ -378,7 +397,7 @@ def main():
         "unit": r.get("unit", "Gcell-updates/s"),
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": {"khi": warmup_steps(args.warmup), "ensemble": args.steps}.get(args.workload, args.warmup),
         "ms_per_step": round(r["s_per_step"] * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",

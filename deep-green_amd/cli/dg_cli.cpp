@@ -7,12 +7,19 @@
 //   dg_cli --demo KELVIN_HELMHOLTZ --tout 0.1 --total-time 1.0 --max-steps 100
 //          [--precision fp32|fp64|fp32_bulk] [--math exact|fast]
 //          [--limiter none|minmod|vanleer] [--out DIR] [--no-dump] [--quiet]
+//          [--restart DIR/state.NNNNN]
 //
 // Dumps: DIR/state.NNNNN/ (main.rs:58-60) with one .npy file per dataset of
-// the reference's HDF5 layout (io.rs:5-196): primitives.{rho,vx,vy,vz,pressure}
-// and conserved.{mass,momentum_x,momentum_y,momentum_z,energy}, float64, full
-// ghosted shape (nx+4, ny+4, nz+4).  The reference's gradients and fluxes
-// groups are not written: the fused step never materializes them.
+// the reference's HDF5 layout (io.rs:5-196), float64, full ghosted shape:
+// primitives.{rho,vx,vy,vz,pressure} (those of the start of the dumped step,
+// as the reference grid holds them after compute_step), conserved.{mass,
+// momentum_x,momentum_y,momentum_z,energy} (the advanced state; ghost cells
+// follow the boundary maps), gradients.{rho,vx,vy,vz,pressure} and
+// fluxes.{mass,momentum_x,momentum_y,momentum_z,energy} of the dumped step,
+// shape (nx+4, ny+4, nz+4, 3) with the axis last.  libhdf5 is not available;
+// .npy is read by numpy and every HDF5-side tool chain through it.  Each dump
+// also holds meta.txt: the loop clock after the dumped step (step, tnow,
+// wtime, wstep), from which --restart continues the run exactly.
 #include <sys/stat.h>
 
 #include <chrono>
@@ -287,16 +294,16 @@ bool find_demo(const std::string &name, Demo &d)
     return true;
 }
 
-bool write_npy(const std::string &path, const std::vector<double> &v, int nx, int ny, int nz)
+bool write_npy(const std::string &path, const std::vector<double> &v, const std::vector<int> &shape)
 {
     FILE *fp = std::fopen(path.c_str(), "wb");
     if (!fp) return false;
-    char hdr[256];
-    int len = std::snprintf(hdr, sizeof(hdr), "{'descr': '<f8', 'fortran_order': False, 'shape': (%d, %d, %d), }",
-                            nx, ny, nz);
-    int total = 10 + len + 1;
+    std::string dims;
+    for (size_t i = 0; i < shape.size(); ++i) dims += std::to_string(shape[i]) + ", ";
+    const std::string hdr = "{'descr': '<f8', 'fortran_order': False, 'shape': (" + dims + "), }";
+    const int total = 10 + (int)hdr.size() + 1;
     const int pad = (64 - total % 64) % 64;
-    std::string h(hdr, len);
+    std::string h = hdr;
     h.append(pad, ' ');
     h.push_back('\n');
     const unsigned short hl = (unsigned short)h.size();
@@ -307,31 +314,123 @@ bool write_npy(const std::string &path, const std::vector<double> &v, int nx, in
     return std::fclose(fp) == 0 && ok;
 }
 
-// save_to_disk (io.rs:5-196): primitives and conserved, f64, ghosted
-int dump(dg_ctx *ctx, const Grid &shape, int precision, const std::string &dir)
+// A float64 C-order .npy file written by write_npy; false unless its shape is `shape`.
+bool read_npy(const std::string &path, const std::vector<int> &shape, std::vector<double> &v)
 {
-    static const char *names[2][5] = {{"rho", "vx", "vy", "vz", "pressure"},
-                                      {"mass", "momentum_x", "momentum_y", "momentum_z", "energy"}};
-    static const char *groups[2] = {"primitives", "conserved"};
+    FILE *fp = std::fopen(path.c_str(), "rb");
+    if (!fp) return false;
+    char magic[8];
+    unsigned short hl = 0;
+    bool ok = std::fread(magic, 1, 8, fp) == 8 && std::memcmp(magic, "\x93NUMPY\x01\x00", 8) == 0 &&
+              std::fread(&hl, 2, 1, fp) == 1;
+    std::string h(hl, ' ');
+    ok = ok && std::fread(&h[0], 1, hl, fp) == hl;
+    std::string dims;
+    for (size_t i = 0; i < shape.size(); ++i) dims += std::to_string(shape[i]) + ", ";
+    ok = ok && h.find("'descr': '<f8'") != std::string::npos && h.find("'fortran_order': False") != std::string::npos &&
+         h.find("'shape': (" + dims + ")") != std::string::npos;
+    size_t n = 1;
+    for (int d : shape) n *= (size_t)d;
+    v.assign(n, 0.0);
+    ok = ok && std::fread(v.data(), sizeof(double), n, fp) == n;
+    std::fclose(fp);
+    return ok;
+}
+
+struct Clock {
+    long long step = 0, wstep = 0;
+    float tnow = 0.0f, wtime = 0.0f;
+};
+
+const char *kPrimNames[5] = {"rho", "vx", "vy", "vz", "pressure"};
+const char *kConsNames[5] = {"mass", "momentum_x", "momentum_y", "momentum_z", "energy"};
+
+// host arrays of the library's element type, as float64 for the .npy files
+std::vector<double> to_f64(const std::vector<char> &b, size_t n, size_t el)
+{
+    std::vector<double> v(n);
+    for (size_t i = 0; i < n; ++i)
+        v[i] = el == 8 ? reinterpret_cast<const double *>(b.data())[i] : (double)reinterpret_cast<const float *>(b.data())[i];
+    return v;
+}
+
+// One step whose result is dumped: save_to_disk (io.rs:5-196) after
+// compute_step (main.rs:56-62) -- the four groups, f64, ghosted -- plus the
+// loop clock after the step (`after`) for --restart.
+int step_and_dump(dg_ctx *ctx, const Grid &shape, int precision, const std::string &dir, double *dt,
+                  const Clock &before)
+{
     mkdir(dir.c_str(), 0755);
     const size_t n = (size_t)shape.nx * shape.ny * shape.nz;
     const size_t el = precision == DG_FP64 ? 8 : 4;
-    std::vector<std::vector<char>> buf(5, std::vector<char>(n * el));
-    for (int kind = 0; kind < 2; ++kind) {
-        void *f[5];
-        for (int q = 0; q < 5; ++q) f[q] = buf[q].data();
-        int rc = dg_get_state(ctx, f, kind == 0 ? DG_PRIMITIVE : DG_CONSERVED);
-        if (rc) return rc;
-        for (int q = 0; q < 5; ++q) {
-            std::vector<double> v(n);
-            for (size_t i = 0; i < n; ++i)
-                v[i] = el == 8 ? reinterpret_cast<const double *>(buf[q].data())[i]
-                               : (double)reinterpret_cast<const float *>(buf[q].data())[i];
-            const std::string path = dir + "/" + groups[kind] + "." + names[kind][q] + ".npy";
-            if (!write_npy(path, v, shape.nx, shape.ny, shape.nz)) return DG_ERR_INVALID;
-        }
+    std::vector<std::vector<char>> prim(5, std::vector<char>(n * el)), grad(5, std::vector<char>(3 * n * el)),
+        flux(5, std::vector<char>(3 * n * el)), cons(5, std::vector<char>(n * el));
+    void *pp[5], *pg[5], *pf[5], *pc[5];
+    for (int q = 0; q < 5; ++q) {
+        pp[q] = prim[q].data();
+        pg[q] = grad[q].data();
+        pf[q] = flux[q].data();
+        pc[q] = cons[q].data();
     }
-    return DG_OK;
+    int rc = dg_step_dump(ctx, dt, pp, pg, pf);
+    if (!rc) rc = dg_get_state(ctx, pc, DG_CONSERVED);
+    if (rc) return rc;
+    const std::vector<int> s3 = {shape.nx, shape.ny, shape.nz}, s4 = {shape.nx, shape.ny, shape.nz, 3};
+    for (int q = 0; q < 5; ++q) {
+        if (!write_npy(dir + "/primitives." + kPrimNames[q] + ".npy", to_f64(prim[q], n, el), s3) ||
+            !write_npy(dir + "/conserved." + kConsNames[q] + ".npy", to_f64(cons[q], n, el), s3) ||
+            !write_npy(dir + "/gradients." + kPrimNames[q] + ".npy", to_f64(grad[q], 3 * n, el), s4) ||
+            !write_npy(dir + "/fluxes." + kConsNames[q] + ".npy", to_f64(flux[q], 3 * n, el), s4))
+            return DG_ERR_INVALID;
+    }
+    // the clock at the top of the next loop iteration: the dump resets wtime
+    // and counts wstep, then tnow += dt; wtime += dt; step += 1 (main.rs:61-66)
+    Clock after = before;
+    after.wtime = 0.0f;
+    after.wstep += 1;
+    after.tnow += (float)*dt;
+    after.wtime += (float)*dt;
+    after.step += 1;
+    FILE *fp = std::fopen((dir + "/meta.txt").c_str(), "w");
+    if (!fp) return DG_ERR_INVALID;
+    std::fprintf(fp, "step %lld\ntnow %.9g\nwtime %.9g\nwstep %lld\n", after.step, (double)after.tnow,
+                 (double)after.wtime, after.wstep);
+    return std::fclose(fp) == 0 ? DG_OK : DG_ERR_INVALID;
+}
+
+// --restart: the conserved state and the loop clock of a dump.
+int load_dump(dg_ctx *ctx, const Grid &shape, int precision, const std::string &dir, Clock &clk)
+{
+    const size_t n = (size_t)shape.nx * shape.ny * shape.nz;
+    const size_t el = precision == DG_FP64 ? 8 : 4;
+    const std::vector<int> s3 = {shape.nx, shape.ny, shape.nz};
+    std::vector<std::vector<char>> buf(5, std::vector<char>(n * el));
+    const void *f[5];
+    for (int q = 0; q < 5; ++q) {
+        std::vector<double> v;
+        if (!read_npy(dir + "/conserved." + kConsNames[q] + ".npy", s3, v)) {
+            std::fprintf(stderr, "cannot read %s/conserved.%s.npy with shape (%d, %d, %d)\n", dir.c_str(),
+                         kConsNames[q], shape.nx, shape.ny, shape.nz);
+            return DG_ERR_INVALID;
+        }
+        for (size_t i = 0; i < n; ++i) {
+            if (el == 8) reinterpret_cast<double *>(buf[q].data())[i] = v[i];
+            else reinterpret_cast<float *>(buf[q].data())[i] = (float)v[i];
+        }
+        f[q] = buf[q].data();
+    }
+    FILE *fp = std::fopen((dir + "/meta.txt").c_str(), "r");
+    double tnow = 0, wtime = 0;
+    const bool ok = fp && std::fscanf(fp, "step %lld\ntnow %lf\nwtime %lf\nwstep %lld", &clk.step, &tnow, &wtime,
+                                      &clk.wstep) == 4;
+    if (fp) std::fclose(fp);
+    if (!ok) {
+        std::fprintf(stderr, "cannot read %s/meta.txt\n", dir.c_str());
+        return DG_ERR_INVALID;
+    }
+    clk.tnow = (float)tnow;
+    clk.wtime = (float)wtime;
+    return dg_set_state(ctx, f, DG_CONSERVED);
 }
 
 void usage()
@@ -340,6 +439,7 @@ void usage()
                  "usage: dg_cli --demo NAME --tout T --total-time T --max-steps N\n"
                  "              [--precision fp32|fp64|fp32_bulk] [--math exact|fast]\n"
                  "              [--limiter none|minmod|vanleer] [--out DIR] [--no-dump] [--quiet]\n"
+                 "              [--restart DIR/state.NNNNN] (continue from a dump of the same set-up)\n"
                  "       dg_cli --config sod64|blast256|khi512 --steps N [--seed S] [options above]\n"
                  "              (gamma 1.4, CFL 0.4, no gravity; dumps only with --out: after the\n"
                  "              first step, then every --tout; default precision fp64, fp64, fp32)\n"
@@ -351,7 +451,7 @@ void usage()
 
 int main(int argc, char **argv)
 {
-    std::string demo, config, out = ".", precision, math = "exact", limiter = "none";
+    std::string demo, config, out = ".", precision, math = "exact", limiter = "none", restart;
     float tout = -1, total_time = -1;
     long long max_steps = -1;
     uint64_t seed = 1;
@@ -375,7 +475,16 @@ int main(int argc, char **argv)
         else if (k == "--config") config = val();
         else if (k == "--steps") max_steps = std::strtoll(val().c_str(), nullptr, 10);
         else if (k == "--seed") seed = std::strtoull(val().c_str(), nullptr, 10);
+        else if (k == "--restart") restart = val();
         else { usage(); return 2; }
+    }
+    const bool known = (precision.empty() || precision == "fp32" || precision == "fp64" || precision == "fp32_bulk") &&
+                       (math == "exact" || math == "fast") &&
+                       (limiter == "none" || limiter == "minmod" || limiter == "vanleer");
+    if (!known) {
+        std::fprintf(stderr, "unknown --precision, --math or --limiter value\n");
+        usage();
+        return 2;
     }
     Demo d;
     std::string default_precision = "fp32";
@@ -418,51 +527,67 @@ int main(int argc, char **argv)
     int rc = dg_create(&cfg, &ctx);
     if (rc) { std::fprintf(stderr, "dg_create failed (%d)\n", rc); return 1; }
     Grid g(d.n[0], d.n[1], d.n[2], d.ds);
-    d.init(g);
-    std::printf("Running simulation %s: %dx%dx%d cells, ds %g.\n", demo.c_str(), d.n[0], d.n[1], d.n[2], (double)d.ds);
-    // state upload (calc_conserved of main.rs:50 happens in the library)
-    std::vector<std::vector<char>> host(5);
-    const size_t n = (size_t)g.nx * g.ny * g.nz, el = cfg.precision == DG_FP64 ? 8 : 4;
-    const void *f[5];
-    for (int q = 0; q < 5; ++q) {
-        host[q].resize(n * el);
-        for (size_t i = 0; i < n; ++i) {
-            if (el == 8) reinterpret_cast<double *>(host[q].data())[i] = g.f[q][i];
-            else reinterpret_cast<float *>(host[q].data())[i] = g.f[q][i];
+    Clock clk;
+    if (!restart.empty()) {
+        // continue a run from one of its dumps (state and loop clock)
+        if ((rc = load_dump(ctx, g, cfg.precision, restart, clk))) {
+            std::fprintf(stderr, "restart from %s failed (%d)\n", restart.c_str(), rc);
+            return 1;
         }
-        f[q] = host[q].data();
+        std::printf("Restarting simulation %s from %s: step %lld, time %g.\n", demo.c_str(), restart.c_str(),
+                    clk.step, (double)clk.tnow);
+    } else {
+        d.init(g);
+        std::printf("Running simulation %s: %dx%dx%d cells, ds %g.\n", demo.c_str(), d.n[0], d.n[1], d.n[2],
+                    (double)d.ds);
+        // state upload (calc_conserved of main.rs:50 happens in the library)
+        std::vector<std::vector<char>> host(5);
+        const size_t n = (size_t)g.nx * g.ny * g.nz, el = cfg.precision == DG_FP64 ? 8 : 4;
+        const void *f[5];
+        for (int q = 0; q < 5; ++q) {
+            host[q].resize(n * el);
+            for (size_t i = 0; i < n; ++i) {
+                if (el == 8) reinterpret_cast<double *>(host[q].data())[i] = g.f[q][i];
+                else reinterpret_cast<float *>(host[q].data())[i] = g.f[q][i];
+            }
+            f[q] = host[q].data();
+        }
+        if ((rc = dg_set_state(ctx, f, DG_PRIMITIVE))) {
+            std::fprintf(stderr, "dg_set_state: %s\n", dg_last_error(ctx));
+            return 1;
+        }
     }
-    if ((rc = dg_set_state(ctx, f, DG_PRIMITIVE))) {
-        std::fprintf(stderr, "dg_set_state: %s\n", dg_last_error(ctx));
-        return 1;
-    }
-    // compute() of main.rs:38-68
-    float tnow = 0.0f, dt = 0.5f, wtime = 0.0f;
-    long long step = 0, wstep = 0;
-    while (!(tnow > total_time || step >= max_steps)) {
-        if (!quiet) std::printf("Computing step %lld and time %g with dt %g. \n", step, (double)tnow, (double)dt);
+    // compute() of main.rs:38-68 (dt starts at 0.5 only for the first print)
+    float dt = 0.5f;
+    while (!(clk.tnow > total_time || clk.step >= max_steps)) {
+        if (!quiet) std::printf("Computing step %lld and time %g with dt %g. \n", clk.step, (double)clk.tnow, (double)dt);
         double dtd = 0;
-        if ((rc = dg_step(ctx, &dtd))) {
+        // the dump test of main.rs:57 depends only on wtime and wstep, which
+        // the step does not change: a dumped step is taken by dg_step_dump
+        const bool dump_now = clk.wtime > tout || clk.wstep == 0;
+        if (dump_now && do_dump) {
+            char name[32];
+            std::snprintf(name, sizeof(name), "state.%05lld", clk.wstep);
+            if ((rc = step_and_dump(ctx, g, cfg.precision, out + "/" + name, &dtd, clk))) {
+                std::fprintf(stderr, "dump failed (%d): %s\n", rc, dg_last_error(ctx));
+                return 1;
+            }
+        } else if ((rc = dg_step(ctx, &dtd))) {
             std::fprintf(stderr, "dg_step: %s\n", dg_last_error(ctx));
             return 1;
         }
         dt = (float)dtd;
-        if (wtime > tout || wstep == 0) {
-            char name[32];
-            std::snprintf(name, sizeof(name), "state.%05lld", wstep);
-            if (do_dump && (rc = dump(ctx, g, cfg.precision, out + "/" + name))) {
-                std::fprintf(stderr, "dump failed (%d)\n", rc);
-                return 1;
-            }
-            wtime = 0.0f;
-            wstep += 1;
+        if (dump_now) {
+            clk.wtime = 0.0f;
+            clk.wstep += 1;
         }
-        tnow += dt;
-        wtime += dt;
-        step += 1;
+        clk.tnow += dt;
+        clk.wtime += dt;
+        clk.step += 1;
     }
     dg_destroy(ctx);
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    std::printf("Simulation done in %.3fs! (%lld steps, t = %g, %lld dumps)\n", secs, step, (double)tnow, wstep);
+    std::printf("Simulation done in %.3fs! (%lld steps, t = %g, %lld dumps)\n", secs, clk.step, (double)clk.tnow,
+                clk.wstep);
     return 0;
 }

@@ -165,17 +165,16 @@ int dg_create(const dg_config *cfg, dg_ctx **out) { return dg::create_ctx(cfg, n
 
 namespace dg {
 
-// Workgroup tiling of a context: tile_y rows (4, 8, 16; FP64 4, 8), chunk_x
+// Workgroup tiling of a grid: tile_y rows (4, 8, 16; FP64 4, 8), chunk_x
 // x-planes per workgroup (0: the longest chunks that still give ~512
 // workgroups -- two resident per CU of a 256-CU device, one wave of
 // workgroups -- and at least 8 planes; long chunks re-read fewer halo
-// planes).  Drops captured graphs (they bake in the grid).
-int set_tiling(dg_ctx *c, int tile_y, int chunk_x)
+// planes).
+int tile_args(StepArgs &a, int prec, int tile_y, int chunk_x, int *chunk_out)
 {
     if (tile_y != 4 && tile_y != 8 && tile_y != 16) return DG_ERR_INVALID;
-    if (c->prec == DG_FP64 && tile_y > 8) return DG_ERR_INVALID;
+    if (prec == DG_FP64 && tile_y > 8) return DG_ERR_INVALID;
     if (chunk_x < 0) return DG_ERR_INVALID;
-    dg::StepArgs a = c->args;
     a.nby = (a.n[1] + tile_y - 1) / tile_y;
     a.nbz = (a.n[2] + 63) / 64;
     int lx = chunk_x;
@@ -191,6 +190,17 @@ int set_tiling(dg_ctx *c, int tile_y, int chunk_x)
     if (nb > INT_MAX / 2) return DG_ERR_INVALID;
     a.nblocks = (int)nb;
     a.swizzle = (a.nblocks % 8 == 0) ? 1 : 0;
+    if (chunk_out) *chunk_out = lx;
+    return DG_OK;
+}
+
+// Tiling of a context; drops captured graphs (they bake in the grid).
+int set_tiling(dg_ctx *c, int tile_y, int chunk_x)
+{
+    dg::StepArgs a = c->args;
+    int lx = 0;
+    const int rc = tile_args(a, c->prec, tile_y, chunk_x, &lx);
+    if (rc) return rc;
     c->args = a;
     c->tile_y = tile_y;
     c->chunk_x = lx;
@@ -199,6 +209,61 @@ int set_tiling(dg_ctx *c, int tile_y, int chunk_x)
             (void)hipGraphExecDestroy(c->graph[b]);
             c->graph[b] = nullptr;
         }
+    return DG_OK;
+}
+
+// Validates a configuration and derives the kernel arguments (grid,
+// boundaries, constants, default tiling) from it.
+int make_args(const dg_config *cfg, StepArgs *out, int *tile_y_out, int *heat_mask_out)
+{
+    if (!cfg || !out) return DG_ERR_INVALID;
+    if (cfg->nghosts != 2) return DG_ERR_INVALID;
+    for (int d = 0; d < 3; ++d)
+        if (cfg->n[d] < 2 || cfg->n[d] > (1 << 20)) return DG_ERR_INVALID;
+    for (int i = 0; i < 6; ++i)
+        if (cfg->bc[i] < 0 || cfg->bc[i] > 3) return DG_ERR_INVALID;
+    for (int d = 0; d < 3; ++d)  // periodic faces pair up (grid.rs:55-63)
+        if ((cfg->bc[2 * d] == DG_PERIODIC) != (cfg->bc[2 * d + 1] == DG_PERIODIC)) return DG_ERR_INVALID;
+    if (!(cfg->ds > 0) || !(cfg->gamma > 1) || !(cfg->cfl > 0)) return DG_ERR_INVALID;
+    if (cfg->precision < 0 || cfg->precision > 2 || cfg->math < 0 || cfg->math > 1) return DG_ERR_INVALID;
+    if (cfg->limiter < 0 || cfg->limiter > 2) return DG_ERR_INVALID;
+    if (cfg->precision == DG_FP64 && cfg->math == DG_MATH_FAST) return DG_ERR_INVALID;
+    const size_t n3 = (size_t)cfg->n[0] * cfg->n[1] * cfg->n[2];
+    // the kernels address a field with 32-bit byte offsets
+    const size_t el_bytes = cfg->precision == DG_FP64 ? sizeof(double) : sizeof(float);
+    if (n3 * el_bytes >= ((size_t)1 << 32)) return DG_ERR_INVALID;
+    const int prec = cfg->precision, fast = cfg->math == DG_MATH_FAST;
+    // default tile height: 8 rows (two workgroups per CU); FP32 exact math
+    // keeps one 16-row workgroup per CU (its register footprint allows no
+    // second one, and the taller tile re-reads fewer halo rows): 29.8 against
+    // 25.6 Gcell/s on a 512^3 periodic grid
+    int ty = cfg->tile_y ? cfg->tile_y : ((prec == DG_FP32 && !fast) ? 16 : 8);
+    if (prec == DG_FP64 && ty > 8) ty = 8;
+
+    StepArgs a{};
+    for (int d = 0; d < 3; ++d) a.n[d] = (int)cfg->n[d];
+    // a conducting wall is a wall for the flow; a lower one carries the heat source
+    int heat_mask = 0;
+    for (int i = 0; i < 6; ++i) a.bc[i] = cfg->bc[i] == DG_CONDUCTING_WALL ? DG_WALL : cfg->bc[i];
+    for (int d = 0; d < 3; ++d)
+        if (cfg->bc[2 * d] == DG_CONDUCTING_WALL) heat_mask |= 1 << d;
+    a.limiter = cfg->limiter;
+    a.gravity = cfg->gravity;
+    a.ds = cfg->ds;
+    {
+        // a power of two whose cube and halves stay normal in float: dividing by
+        // ds, 2 ds or ds^3 is then exactly a multiplication by the reciprocal
+        // (both round the same real number), which the exact kernels use
+        int ex = 0;
+        const double mant = std::frexp(cfg->ds, &ex);
+        a.pow2 = (mant == 0.5 && ex > -40 && ex < 40) ? 1 : 0;
+    }
+    a.gamma = cfg->gamma;
+    a.g = cfg->g;
+    if (tile_args(a, prec, ty, cfg->chunk_x, nullptr) != DG_OK) return DG_ERR_INVALID;
+    *out = a;
+    if (tile_y_out) *tile_y_out = ty;
+    if (heat_mask_out) *heat_mask_out = heat_mask;
     return DG_OK;
 }
 
@@ -230,7 +295,7 @@ int launch_one_step(dg_ctx *c)
     int nparts = c->args.nblocks;
     if (c->heat_mask) {
         // heat through conducting walls, then the next dt from the heated state
-        if (dg::launch_heat(c->prec, c->args, in, out, c->sc, c->heat_t_out, c->heat_v_out, c->heat_rs,
+        if (dg::launch_heat(c->prec, c->fast, c->args, in, out, c->sc, c->heat_t_out, c->heat_v_out, c->heat_rs,
                             c->heat_mask, c->stream))
             return fail(c, DG_ERR_DEVICE, "heat launch");
         const void *u[5] = {out[0], out[1], out[2], out[3], out[4]};
@@ -265,21 +330,9 @@ int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out)
 {
     if (!cfg || !out) return DG_ERR_INVALID;
     *out = nullptr;
-    if (cfg->nghosts != 2) return DG_ERR_INVALID;
-    for (int d = 0; d < 3; ++d)
-        if (cfg->n[d] < 2 || cfg->n[d] > (1 << 20)) return DG_ERR_INVALID;
-    for (int i = 0; i < 6; ++i)
-        if (cfg->bc[i] < 0 || cfg->bc[i] > 3) return DG_ERR_INVALID;
-    for (int d = 0; d < 3; ++d)  // periodic faces pair up (grid.rs:55-63)
-        if ((cfg->bc[2 * d] == DG_PERIODIC) != (cfg->bc[2 * d + 1] == DG_PERIODIC)) return DG_ERR_INVALID;
-    if (!(cfg->ds > 0) || !(cfg->gamma > 1) || !(cfg->cfl > 0)) return DG_ERR_INVALID;
-    if (cfg->precision < 0 || cfg->precision > 2 || cfg->math < 0 || cfg->math > 1) return DG_ERR_INVALID;
-    if (cfg->limiter < 0 || cfg->limiter > 2) return DG_ERR_INVALID;
-    if (cfg->precision == DG_FP64 && cfg->math == DG_MATH_FAST) return DG_ERR_INVALID;
-    const size_t n3 = (size_t)cfg->n[0] * cfg->n[1] * cfg->n[2];
-    // the kernels address a field with 32-bit byte offsets
-    const size_t el_bytes = cfg->precision == DG_FP64 ? sizeof(double) : sizeof(float);
-    if (n3 * el_bytes >= ((size_t)1 << 32)) return DG_ERR_INVALID;
+    dg::StepArgs args{};
+    int ty = 8, heat_mask = 0;
+    if (make_args(cfg, &args, &ty, &heat_mask) != DG_OK) return DG_ERR_INVALID;
 
     dg_ctx *c = new (std::nothrow) dg_ctx();
     if (!c) return DG_ERR_ALLOC;
@@ -289,35 +342,12 @@ int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out)
     c->prec = cfg->precision;
     c->fast = cfg->math == DG_MATH_FAST;
     c->el = c->prec == DG_FP64 ? sizeof(double) : sizeof(float);
-    c->n3 = n3;
-    // default tile height: 8 rows (two workgroups per CU); FP32 exact math
-    // keeps one 16-row workgroup per CU (its register footprint allows no
-    // second one, and the taller tile re-reads fewer halo rows): 29.8 against
-    // 25.6 Gcell/s on a 512^3 periodic grid
-    int ty = cfg->tile_y ? cfg->tile_y : ((c->prec == DG_FP32 && !c->fast) ? 16 : 8);
-    if (c->prec == DG_FP64 && ty > 8) ty = 8;
-
-    dg::StepArgs &a = c->args;
-    for (int d = 0; d < 3; ++d) a.n[d] = (int)cfg->n[d];
-    // a conducting wall is a wall for the flow; a lower one carries the heat source
-    for (int i = 0; i < 6; ++i) a.bc[i] = cfg->bc[i] == DG_CONDUCTING_WALL ? DG_WALL : cfg->bc[i];
-    for (int d = 0; d < 3; ++d)
-        if (cfg->bc[2 * d] == DG_CONDUCTING_WALL) c->heat_mask |= 1 << d;
-    a.limiter = cfg->limiter;
-    a.gravity = cfg->gravity;
-    a.ds = cfg->ds;
-    {
-        // a power of two whose cube and halves stay normal in float: dividing by
-        // ds, 2 ds or ds^3 is then exactly a multiplication by the reciprocal
-        // (both round the same real number), which the exact kernels use
-        int ex = 0;
-        const double mant = std::frexp(cfg->ds, &ex);
-        a.pow2 = (mant == 0.5 && ex > -40 && ex < 40) ? 1 : 0;
-    }
-    a.gamma = cfg->gamma;
-    a.g = cfg->g;
-    c->nparts_reduce = dg::reduce_blocks(a);
+    c->n3 = (size_t)cfg->n[0] * cfg->n[1] * cfg->n[2];
+    c->args = args;
+    c->heat_mask = heat_mask;
+    c->nparts_reduce = dg::reduce_blocks(args);
     if (set_tiling(c, ty, cfg->chunk_x) != DG_OK) { delete c; return DG_ERR_INVALID; }
+    const size_t n3 = c->n3;
 
     hipError_t e = hipSetDevice(cfg->device);
     if (e != hipSuccess) { delete c; return DG_ERR_DEVICE; }
@@ -367,6 +397,7 @@ void dg_destroy(dg_ctx *c)
     if (c->partials) (void)hipFree(c->partials);
     if (c->sc) (void)hipFree(c->sc);
     if (c->scratch) (void)hipFree(c->scratch);
+    if (c->dumpbuf) (void)hipFree(c->dumpbuf);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->stream && c->owns_stream) (void)hipStreamDestroy(c->stream);
@@ -571,7 +602,7 @@ int dg_time_steps(dg_ctx *c, int64_t nsteps, double *ms_total, double *ms_step_k
         int nparts = c->args.nblocks;
         if (c->heat_mask) {   // conducting walls: heat, then the dt of the heated state
             const void *u2[5] = {out[0], out[1], out[2], out[3], out[4]};
-            if (dg::launch_heat(c->prec, c->args, in, out, c->sc, c->heat_t_out, c->heat_v_out, c->heat_rs,
+            if (dg::launch_heat(c->prec, c->fast, c->args, in, out, c->sc, c->heat_t_out, c->heat_v_out, c->heat_rs,
                                 c->heat_mask, c->stream) ||
                 dg::launch_reduce(c->prec, c->fast, c->args, u2, c->partials, c->nparts_reduce, c->stream))
                 rc = fail(c, DG_ERR_DEVICE, "heat launch");

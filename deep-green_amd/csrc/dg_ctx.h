@@ -26,6 +26,7 @@ struct dg_ctx {
     dg::Scalars *sc_host = nullptr; // pinned mirror
     void *stage = nullptr;          // pinned staging (one field), allocated at first upload
     void *scratch = nullptr;        // device scratch (one state) for primitive downloads
+    void *dumpbuf = nullptr;        // ghosted arrays of dg_step_dump (allocated at the first dump)
     hipStream_t stream = nullptr;
     bool owns_stream = true;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // dg_run replay, by state buffer parity
@@ -39,6 +40,11 @@ struct dg_ctx {
 
 
 namespace dg {
+// Validates a configuration and derives the kernel arguments (default tiling
+// included); tile_y_out / heat_mask_out may be null.
+int make_args(const dg_config *cfg, StepArgs *out, int *tile_y_out, int *heat_mask_out);
+// Workgroup tiling of a grid (tile_y rows, chunk_x planes; 0 = automatic).
+int tile_args(StepArgs &a, int prec, int tile_y, int chunk_x, int *chunk_out);
 int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out);
 int set_tiling(dg_ctx *c, int tile_y, int chunk_x);
 int reserve_partials(dg_ctx *c);

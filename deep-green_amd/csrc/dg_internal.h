@@ -110,8 +110,8 @@ int launch_residual(long long n3, const void *const ut[5], const void *const up[
                     void *const r[5], hipStream_t s);
 int launch_set_dt(Scalars *sc, double dt, hipStream_t s);
 // Heat through conducting lower walls (mask bit 0 x-, 1 y-, 2 z-), added to unew.
-int launch_heat(int prec, const StepArgs &a, const void *const uold[5], void *const unew[5], const Scalars *sc,
-                double t_out, double v_out, double rs, int mask, hipStream_t s);
+int launch_heat(int prec, int fast, const StepArgs &a, const void *const uold[5], void *const unew[5],
+                const Scalars *sc, double t_out, double v_out, double rs, int mask, hipStream_t s);
 // Resident workgroups of the ensemble step kernel per CU (-1 on error).
 int ens_blocks_per_cu(int prec, int fast, int tile_y, int limiter, int waves);
 int step_lds_bytes(int prec, int tile_y);

@@ -275,6 +275,28 @@ struct Consts {
     bool pow2;   // ds a power of two: divisions by ds, 2 ds, ds^3 as exact products
 };
 
+// Constants of one step with time step dt.
+template <typename T, bool FAST>
+__device__ __forceinline__ Consts<T> make_consts(const StepArgs &A, T dt)
+{
+    Consts<T> C;
+    C.ds = T(A.ds);
+    C.gamma = T(A.gamma);
+    C.gm1 = C.gamma - T(1.0);
+    C.igm1 = T(1) / C.gm1;
+    C.V = C.ds * C.ds * C.ds;
+    C.invV = T(1) / C.V;
+    C.two_ds = T(2.0) * C.ds;
+    C.inv_two_ds = T(1) / C.two_ds;
+    C.inv_ds = T(1) / C.ds;
+    C.pow2 = A.pow2 != 0;
+    C.half_ds = FAST ? T(0.25) : C.ds / T(2.);   // fast: slopes are scaled by 2 ds
+    C.G = T(A.g);
+    C.half_dt = FAST ? T(0.5) * dt * C.inv_two_ds : T(0.5) * dt;
+    C.dtds = FAST ? dt * C.ds * C.ds * T(0.5) : dt * C.ds;
+    return C;
+}
+
 // Two-element vector for 8-byte (float) / 16-byte (double) LDS accesses.
 template <typename T> using vec2 = T __attribute__((ext_vector_type(2)));
 
@@ -355,6 +377,38 @@ __device__ __forceinline__ void ld_prims(const T *p, T v[6])
     v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = p[4]; v[5] = T(0);
 }
 
+// Hancock half-step (extrapolate_primitives_t, physics.rs:217-282) of a
+// cell with primitives w, slopes g (g[f*3+d]: field f along axis d) and, in
+// fast math, inverse density ir.
+template <typename T, bool FAST>
+__device__ __forceinline__ void hancock(const T w[5], T ir, const T g[15], const Consts<T> &C, T xtr[5])
+{
+    const T r = w[0], ux = w[1], uy = w[2], uz = w[3], p = w[4];
+    const T drx = g[0], dry = g[1], drz = g[2];
+    const T dxx = g[3], dxy = g[4], dxz = g[5];
+    const T dyx = g[6], dyy = g[7], dyz = g[8];
+    const T dzx = g[9], dzy = g[10], dzz = g[11];
+    const T dpx = g[12], dpy = g[13], dpz = g[14];
+    const T hd = C.half_dt;
+    if constexpr (!FAST) {
+        // physics.rs:251-278, same association
+        (void)ir;
+        xtr[0] = r - hd * (ux * drx + r * dxx + uy * dry + r * dyy + uz * drz + r * dzz);
+        xtr[1] = ux - hd * (ux * dxx + uy * dxy + uz * dxz + (T(1.0) / r) * dpx);
+        xtr[2] = uy - hd * (ux * dyx + uy * dyy + uz * dyz + (T(1.0) / r) * dpy);
+        xtr[3] = uz - hd * (ux * dzx + uy * dzy + uz * dzz + (T(1.0) / r) * dpz);
+        xtr[4] = p - hd * (C.gamma * p * (dxx + dyy + dzz) + ux * dpx + uy * dpy + uz * dpz);
+    } else {
+        // hd here is dt / 2 / (2 ds): slopes are undivided differences
+        const T div = dxx + dyy + dzz;
+        xtr[0] = dfma<T>(-hd, dfma<T>(uz, drz, dfma<T>(uy, dry, dfma<T>(ux, drx, r * div))), r);
+        xtr[1] = dfma<T>(-hd, dfma<T>(ir, dpx, dfma<T>(uz, dxz, dfma<T>(uy, dxy, ux * dxx))), ux);
+        xtr[2] = dfma<T>(-hd, dfma<T>(ir, dpy, dfma<T>(uz, dyz, dfma<T>(uy, dyy, ux * dyx))), uy);
+        xtr[3] = dfma<T>(-hd, dfma<T>(ir, dpz, dfma<T>(uz, dzz, dfma<T>(uy, dzy, ux * dzx))), uz);
+        xtr[4] = dfma<T>(-hd, dfma<T>(uz, dpz, dfma<T>(uy, dpy, dfma<T>(ux, dpx, C.gamma * p * div))), p);
+    }
+}
+
 template <typename T, int TY, bool FAST, int LIM>
 __device__ __forceinline__ void eval_cell(const StepLDS<T, TY> &L, int sm, int s0, int sp, const EvalDesc &d,
                                           const Consts<T> &C, T xtr[5], T g[15])
@@ -379,31 +433,7 @@ __device__ __forceinline__ void eval_cell(const StepLDS<T, TY> &L, int sm, int s
         g[f * 3 + 1] = slope<T, FAST, LIM>(ym[f], c, yp[f], C);
         g[f * 3 + 2] = slope<T, FAST, LIM>(zm[f], c, zp[f], C);
     }
-    const T r = w[0], ux = w[1], uy = w[2], uz = w[3], p = w[4];
-    const T drx = g[0], dry = g[1], drz = g[2];
-    const T dxx = g[3], dxy = g[4], dxz = g[5];
-    const T dyx = g[6], dyy = g[7], dyz = g[8];
-    const T dzx = g[9], dzy = g[10], dzz = g[11];
-    const T dpx = g[12], dpy = g[13], dpz = g[14];
-    const T hd = C.half_dt;
-    if constexpr (!FAST) {
-        // physics.rs:251-278, same association
-        xtr[0] = r - hd * (ux * drx + r * dxx + uy * dry + r * dyy + uz * drz + r * dzz);
-        xtr[1] = ux - hd * (ux * dxx + uy * dxy + uz * dxz + (T(1.0) / r) * dpx);
-        xtr[2] = uy - hd * (ux * dyx + uy * dyy + uz * dyz + (T(1.0) / r) * dpy);
-        xtr[3] = uz - hd * (ux * dzx + uy * dzy + uz * dzz + (T(1.0) / r) * dpz);
-        xtr[4] = p - hd * (C.gamma * p * (dxx + dyy + dzz) + ux * dpx + uy * dpy + uz * dpz);
-    } else {
-        // hd here is dt / 2 / (2 ds): slopes are undivided differences;
-        // the ring carries 1/rho
-        const T ir = cc[5];
-        const T div = dxx + dyy + dzz;
-        xtr[0] = dfma<T>(-hd, dfma<T>(uz, drz, dfma<T>(uy, dry, dfma<T>(ux, drx, r * div))), r);
-        xtr[1] = dfma<T>(-hd, dfma<T>(ir, dpx, dfma<T>(uz, dxz, dfma<T>(uy, dxy, ux * dxx))), ux);
-        xtr[2] = dfma<T>(-hd, dfma<T>(ir, dpy, dfma<T>(uz, dyz, dfma<T>(uy, dyy, ux * dyx))), uy);
-        xtr[3] = dfma<T>(-hd, dfma<T>(ir, dpz, dfma<T>(uz, dzz, dfma<T>(uy, dzy, ux * dzx))), uz);
-        xtr[4] = dfma<T>(-hd, dfma<T>(uz, dpz, dfma<T>(uy, dpy, dfma<T>(ux, dpx, C.gamma * p * div))), p);
-    }
+    hancock<T, FAST>(w, cc[5], g, C, xtr);
     if (!d.walls) return;   // wave-uniform: no lane evaluates a wall ghost
     if (d.mask & 1) {   // wall ghost along y: mirrored xtr, slope on the ghost
         xtr[2] = -xtr[2];
@@ -619,22 +649,7 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
     const uint32_t plane_sz = (uint32_t)ny * (uint32_t)nz;   // n^3 * sizeof(T) < 4 GiB (checked at create)
     const uint32_t col = (uint32_t)yc * (uint32_t)nz + (uint32_t)zc;
 
-    Consts<T> C;
-    C.ds = T(A.ds);
-    C.gamma = T(A.gamma);
-    C.gm1 = C.gamma - T(1.0);
-    C.igm1 = T(1) / C.gm1;
-    C.V = C.ds * C.ds * C.ds;
-    C.invV = T(1) / C.V;
-    C.two_ds = T(2.0) * C.ds;
-    C.inv_two_ds = T(1) / C.two_ds;
-    C.inv_ds = T(1) / C.ds;
-    C.pow2 = A.pow2 != 0;
-    C.half_ds = FAST ? T(0.25) : C.ds / T(2.);   // fast: slopes are scaled by 2 ds
-    C.G = T(A.g);
-    const T dt = T(sc->dt);
-    C.half_dt = FAST ? T(0.5) * dt * C.inv_two_ds : T(0.5) * dt;
-    C.dtds = FAST ? dt * C.ds * C.ds * T(0.5) : dt * C.ds;
+    const Consts<T> C = make_consts<T, FAST>(A, T(sc->dt));
 
     if (!sc->active) {
         // inactive step of a captured run: carry the state through unchanged
@@ -1206,24 +1221,61 @@ __global__ void set_dt_kernel(Scalars *sc, double dt) { sc->dt = dt; }
 // Heat through conducting lower walls (the CUDA variant's source,
 // src/cu/include/kernels.h:561-661): the first interior cell along a
 // conducting lower wall gains dt ds (ds Q/V) of energy, Q = (T_out - t_in) S /
-// (d/kappa + 1/hc), t_in = p / (RS rho) of the wall ghost (the mirror of the
-// cell itself) at the start of the step, S = ds^2, V = ds^3, d 0.008, kappa
-// 0.2, hc = 10 - v_out + 10 sqrt(v_out).  Added after the step, axes in the
-// order x, y, z; one thread per cell of the union of the wall layers.
+// (d/kappa + 1/hc), S = ds^2, V = ds^3, d 0.008, kappa 0.2, hc = 10 - v_out +
+// 10 sqrt(v_out).  t_in = p / (RS rho) of the wall ghost in the state the
+// CUDA flux kernel reads, the half-step extrapolated one (physics.hpp:437-442
+// passes rho_xtr / p_xtr): the ghost's extrapolated state is the mirror of the
+// cell's own (handle_walls on xtr), so t_in comes from the cell's Hancock
+// state, evaluated here exactly as the step kernel evaluates it.  Added after
+// the step, axes in the order x, y, z; one thread per cell of the union of
+// the wall layers.
 struct HeatArgs { double t_out, v_out, rs; int mask; };
 
-template <typename T>
+// Slopes and Hancock state of interior cell (i, j, k) from the conserved
+// state U (neighbours through the boundary maps), as eval_cell computes them.
+template <typename T, bool FAST, int LIM>
+__device__ __forceinline__ void cell_xtr(const StepArgs &A, const Fields<const T> &U, const Consts<T> &C, int i,
+                                         int j, int k, T xtr[5])
+{
+    T w[7][6];   // centre, x-, x+, y-, y+, z-, z+
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        int p[3] = {i, j, k};
+        if (q > 0) p[(q - 1) / 2] += (q & 1) ? -1 : 1;
+        const AxisMap mx = map_axis(p[0], A.n[0], A.bc[0], A.bc[1]);
+        const AxisMap my = map_axis(p[1], A.n[1], A.bc[2], A.bc[3]);
+        const AxisMap mz = map_axis(p[2], A.n[2], A.bc[4], A.bc[5]);
+        const uint32_t idx = ((uint32_t)mx.idx * (uint32_t)A.n[1] + (uint32_t)my.idx) * (uint32_t)A.n[2] + (uint32_t)mz.idx;
+        T u[5];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) u[f] = at(U.f[f], idx);
+        to_prim<T, FAST>(u, C.V, C.invV, C.gm1, w[q], C.pow2);
+        w[q][5] = FAST ? C.V * drcp<T>(u[0]) : T(0);
+        if (mx.flip) w[q][1] = -w[q][1];
+        if (my.flip) w[q][2] = -w[q][2];
+        if (mz.flip) w[q][3] = -w[q][3];
+    }
+    T g[15];
+#pragma unroll
+    for (int f = 0; f < 5; ++f)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) g[f * 3 + a] = slope<T, FAST, LIM>(w[1 + 2 * a][f], w[0][f], w[2 + 2 * a][f], C);
+    hancock<T, FAST>(w[0], w[0][5], g, C, xtr);
+}
+
+template <typename T, bool FAST>
 __global__ void __launch_bounds__(256)
 heat_kernel(const StepArgs A, const Fields<const T> Uold, const Fields<T> Unew, const Scalars *__restrict__ sc,
             const HeatArgs H)
 {
     if (!sc->active) return;
-    const T ds = T(A.ds), gm1 = T(A.gamma) - T(1.0);
+    const T dt = T(sc->dt);
+    const Consts<T> C = make_consts<T, FAST>(A, dt);
+    const T ds = T(A.ds);
     const T S = ds * ds, V = ds * ds * ds;
     const T vout = T(H.v_out);
     const T hc = T(10.0) - vout + T(10.0) * dsqrt<T>(vout);
     const T dk = T(0.008) / T(0.2);
-    const T dt = T(sc->dt);
     const int n0 = A.n[0], n1 = A.n[1], n2 = A.n[2];
     // cells of the union: layer x (i = 0), then y (j = 0, i > 0 if x is a wall layer), then z
     const long long lx = (H.mask & 1) ? (long long)n1 * n2 : 0;
@@ -1239,10 +1291,11 @@ heat_kernel(const StepArgs A, const Fields<const T> Uold, const Fields<T> Unew, 
         if (e >= lx && (H.mask & 1) && i == 0) continue;
         if (e >= lx + ly && (H.mask & 2) && j == 0) continue;
         const long long c = ((long long)i * n1 + j) * n2 + k;
-        T u[5], w[5];
-        for (int f = 0; f < 5; ++f) u[f] = Uold.f[f][c];
-        to_prim<T, false>(u, V, T(1) / V, gm1, w);
-        const T t_in = w[4] / (T(H.rs) * w[0]);
+        T xtr[5];
+        if (A.limiter == LIM_MINMOD) cell_xtr<T, FAST, LIM_MINMOD>(A, Uold, C, i, j, k, xtr);
+        else if (A.limiter == LIM_VANLEER) cell_xtr<T, FAST, LIM_VANLEER>(A, Uold, C, i, j, k, xtr);
+        else cell_xtr<T, FAST, LIM_NONE>(A, Uold, C, i, j, k, xtr);
+        const T t_in = xtr[4] / (T(H.rs) * xtr[0]);
         const T Q = (T(H.t_out) - t_in) * (S / (dk + T(1.0) / hc));
         const T dE = dt * ds * ((Q / V) * ds);
         T E = Unew.f[4][c];
@@ -1460,17 +1513,20 @@ int launch_set_dt(Scalars *sc, double dt, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_heat(int prec, const StepArgs &a, const void *const uold[5], void *const unew[5], const Scalars *sc,
-                double t_out, double v_out, double rs, int mask, hipStream_t s)
+int launch_heat(int prec, int fast, const StepArgs &a, const void *const uold[5], void *const unew[5],
+                const Scalars *sc, double t_out, double v_out, double rs, int mask, hipStream_t s)
 {
     HeatArgs H{t_out, v_out, rs, mask};
     const long long cells = (long long)a.n[1] * a.n[2] + (long long)a.n[0] * a.n[2] + (long long)a.n[0] * a.n[1];
     const int blocks = (int)std::min<long long>((cells + 255) / 256, 2048);
     if (prec == 0)
-        hipLaunchKernelGGL(heat_kernel<double>, dim3(blocks), dim3(256), 0, s, a, cfields<double>(uold),
+        hipLaunchKernelGGL((heat_kernel<double, false>), dim3(blocks), dim3(256), 0, s, a, cfields<double>(uold),
                            mfields<double>(unew), sc, H);
+    else if (fast)
+        hipLaunchKernelGGL((heat_kernel<float, true>), dim3(blocks), dim3(256), 0, s, a, cfields<float>(uold),
+                           mfields<float>(unew), sc, H);
     else
-        hipLaunchKernelGGL(heat_kernel<float>, dim3(blocks), dim3(256), 0, s, a, cfields<float>(uold),
+        hipLaunchKernelGGL((heat_kernel<float, false>), dim3(blocks), dim3(256), 0, s, a, cfields<float>(uold),
                            mfields<float>(unew), sc, H);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }

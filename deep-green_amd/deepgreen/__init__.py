@@ -33,7 +33,7 @@ STATUS = {0: "ok", -1: "invalid argument", -2: "device error", -3: "allocation f
 
 # every entry point declared in include/deepgreen.h
 EXPORTS = ("dg_config_default", "dg_create", "dg_destroy", "dg_last_error", "dg_set_state",
-           "dg_get_state", "dg_step", "dg_run", "dg_reset_clock", "dg_conservation_sums",
+           "dg_get_state", "dg_step", "dg_step_dump", "dg_run", "dg_reset_clock", "dg_conservation_sums",
            "dg_next_dt", "dg_device_state", "dg_time_steps", "dg_kernel_info", "dg_device_count", "dg_set_heat",
            "dg_device_info",
            "dg_ensemble_create", "dg_ensemble_destroy", "dg_ensemble_last_error", "dg_ensemble_set_state",
@@ -92,6 +92,7 @@ def load_library(path=None):
     L.dg_set_state.argtypes = [vp, P(vp), i32]
     L.dg_get_state.argtypes = [vp, P(vp), i32]
     L.dg_step.argtypes = [vp, P(d)]
+    L.dg_step_dump.argtypes = [vp, P(d), P(vp), P(vp), P(vp)]
     L.dg_run.argtypes = [vp, i64, d, P(d), P(i64)]
     L.dg_reset_clock.argtypes = [vp]
     L.dg_conservation_sums.argtypes = [vp, P(d)]
@@ -268,6 +269,21 @@ class Solver:
         dt = ctypes.c_double()
         self._check(self._L.dg_step(self._h, ctypes.byref(dt)), "dg_step")
         return dt.value
+
+    def compute_step_dump(self):
+        """One step that also returns the reference grid's other dumped
+        arrays (dg_step_dump): (dt, primitives of the step's start [5 arrays
+        of the ghosted shape], gradients and fluxes [5 arrays of the ghosted
+        shape + (3,)])."""
+        dt = ctypes.c_double()
+        sh = self.sim.shape
+        prim = [np.empty(sh, dtype=self.sim.dtype) for _ in range(5)]
+        grad = [np.empty(sh + (3,), dtype=self.sim.dtype) for _ in range(5)]
+        flux = [np.empty(sh + (3,), dtype=self.sim.dtype) for _ in range(5)]
+        ptr = lambda arrs: (ctypes.c_void_p * 5)(*[a.ctypes.data for a in arrs])  # noqa: E731
+        self._check(self._L.dg_step_dump(self._h, ctypes.byref(dt), ptr(prim), ptr(grad), ptr(flux)),
+                    "dg_step_dump")
+        return dt.value, prim, grad, flux
 
     def set_heat(self, t_out_kelvin=273.0, v_out=0.0, rs=287.0):
         """Constants of the conducting-wall heat source (dg_set_heat)."""

@@ -106,6 +106,21 @@ int dg_get_state(dg_ctx *ctx, void *const f[5], int kind);
  * current state (returned in *dt_used), advances the state by one step. */
 int dg_step(dg_ctx *ctx, double *dt_used);
 
+/* dg_step that also returns what the reference's grid holds after
+ * compute_step besides the conserved state, i.e. the groups its state dump
+ * writes (io.rs:5-196, save_to_disk after compute_step, main.rs:56-62):
+ * prim[5] the primitives of the step's start (rho, vx, vy, vz, p; ghosts
+ * filled, grid.rs:121-135), grad[5] the gradients of rho, vx, vy, vz, p and
+ * flux[5] the fluxes of mass, momentum x/y/z, energy (physics.rs:198-215,
+ * 284-409, after their ghost updates, grid.rs:153-167).  Host arrays in the
+ * reference layout: prim (n+4)^3, grad and flux (n+4)^3 x 3 with the axis
+ * last (Array4, grid.rs:79-109).  Any pointer may be NULL.  The arrays are
+ * computed by separate dump kernels in the reference's arithmetic (exact
+ * math, whatever the context's setting) from the state the step starts from;
+ * the first call allocates 40 ghosted arrays on the device, kept until
+ * dg_destroy.  Fluxes do not include the conducting-wall heat source. */
+int dg_step_dump(dg_ctx *ctx, double *dt_used, void *const prim[5], void *const grad[5], void *const flux[5]);
+
 /* The loop of main.rs:51-67 without output: steps while
  * !(t > t_end || steps >= max_steps), accumulating t += dt in the precision's
  * time type (float for DG_FP32, as the reference).  Runs from a captured HIP

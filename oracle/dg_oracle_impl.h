@@ -464,8 +464,11 @@ void FN(dgo_add_fluxes)(CTX *c, R dt)
 /* Conducting-wall heat source of the CUDA variant (src/cu/include/kernels.h:
  * 561-661), restated: for a conducting lower wall along axis d, the energy
  * flux through the wall face of the first interior cell gains Q/V, with
- * t_in = p / (RS rho) of the wall ghost (index 1, the mirror of the first
- * interior cell, from the primitives of the step's start), hc = 10 - v_out +
+ * t_in = p / (RS rho) of the wall ghost (index 1) in the state the CUDA flux
+ * kernel reads, the half-step extrapolated one (physics.hpp:437-442 passes
+ * rho_xtr and p_xtr as its rho and p): here the xtr arrays after their ghost
+ * fill, where the ghost is the mirror of the first interior cell (handle_walls
+ * on xtr, grid.rs:137-151), hc = 10 - v_out +
  * 10 sqrt(v_out), S = ds^2, V = ds^3, Q = (T_out - t_in) S / (d/kappa +
  * 1/hc), d = 0.008, kappa = 0.2.  Through the update (F scaled by ds,
  * U -= dt ds dF; physics.rs:401-405, 539-593) the first interior cell's
@@ -492,7 +495,7 @@ void FN(dgo_heat)(CTX *c, R dt)
                     int64_t gq[3] = {i, j, k};
                     gq[d] = 1;   /* the wall ghost */
                     const int64_t gi = I3(c, gq[0], gq[1], gq[2]);
-                    const R t_in = c->prim[4][gi] / ((R)c->p.rs * c->prim[0][gi]);
+                    const R t_in = c->xtr[4][gi] / ((R)c->p.rs * c->xtr[0][gi]);
                     const R Q = ((R)c->p.t_out - t_in) * (S / (dk + (R)1.0 / hc));
                     c->cons[4][I3(c, i, j, k)] += dt * ds * ((Q / V) * ds);
                 }

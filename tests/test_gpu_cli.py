@@ -2,9 +2,12 @@
 
 The loop and the dump cadence follow main.rs:38-68 (a dump after the first
 step and whenever wtime > tout); each dump holds the reference's HDF5
-datasets (io.rs) as .npy files.  One step of the library between two dumps
-is checked bit for bit against the oracle started from the first dump
-(FP32 exact math, the reference's constants, walls and gravity)."""
+datasets (io.rs) as .npy files.  For every demo, the library's step between
+two dumps and the dumped arrays are checked bit for bit against the oracle
+started from the first dump (FP32 exact math, the reference's constants,
+walls and gravity); a restarted run is checked against an uninterrupted one.
+The demos' initial conditions themselves (init.rs) are parity unpinned: the
+reference's Rust cannot run here."""
 import os
 import subprocess
 
@@ -30,28 +33,85 @@ def load(dirpath, group, names):
     return [np.load(os.path.join(dirpath, f"{group}.{n}.npy")) for n in names]
 
 
-@pytest.mark.parametrize("demo, bcs, gravity, n, ds", [
-    ("RAYLEIGH_TAYLOR", [0, 0, 0, 0, 2, 2], True, (100, 2, 300), 2.0 * 0.0025),
-    ("SHOCK_TUBE", [2, 2, 0, 0, 0, 0], False, (5000, 2, 2), 0.0002),
-])
-def test_cli_dumps_follow_oracle(tmp_path, demo, bcs, gravity, n, ds):
-    out = run_cli(["--demo", demo, "--tout", "0", "--total-time", "100", "--max-steps", "3", "--out", str(tmp_path)],
+PRIMS = ("rho", "vx", "vy", "vz", "pressure")
+
+# the demos of main.rs:79-108 as init.rs sets them up: interior size, ds,
+# boundaries (x-, x+, y-, y+, z-, z+), gravity
+DEMOS = {
+    "KELVIN_HELMHOLTZ": ((256, 2, 128), 1.0 / 256, [0, 0, 0, 0, 0, 0], False),
+    "RAYLEIGH_TAYLOR": ((100, 2, 300), 2.0 * 0.0025, [0, 0, 0, 0, 2, 2], True),
+    "RAYLEIGH_BERNARD": ((64, 64, 64), 1.0 / 64, [2, 2, 2, 2, 2, 2], True),
+    "EXPLOSION": ((128, 2, 128), 1.0 / 128, [2, 2, 0, 0, 2, 2], False),
+    "RELAXATION": ((30, 30, 30), 0.1, [2, 2, 2, 2, 2, 1], True),
+    "SHOCK_TUBE": ((5000, 2, 2), 0.0002, [2, 2, 0, 0, 0, 0], False),
+    "THERMAL_RISING_BUBBLE": ((150, 2, 512), 1.0, [2, 2, 0, 0, 2, 1], True),
+    "THERMAL_RISING_BUBBLES": ((128, 2, 500), 2.0, [2, 2, 0, 0, 2, 1], True),
+}
+
+
+def demo_oracle(demo):
+    n, ds, bcs, gravity = DEMOS[demo]
+    return Oracle(n, np.float32(ds), bcs, precision="f32", gamma=5.0 / 3.0, cfl=0.2, g=-0.1, gravity=gravity,
+                  gravity_k_shift=0)
+
+
+@pytest.mark.parametrize("demo", sorted(DEMOS))
+def test_cli_demo_dumps_follow_oracle(tmp_path, demo):
+    """Every demo at its reference size: the step between two dumps equals the
+    oracle's step bit for bit, and the dump holds what the reference grid
+    holds after compute_step (io.rs:5-196): the conserved state after the
+    step, the primitives of the step's start and the step's gradients and
+    fluxes, ghost cells included (those of the conserved state follow the
+    boundary maps)."""
+    n = DEMOS[demo][0]
+    out = run_cli(["--demo", demo, "--tout", "0", "--total-time", "1e9", "--max-steps", "2", "--out", str(tmp_path)],
                   str(tmp_path))
-    assert "Computing step 2" in out
+    assert "Computing step 1" in out
     dumps = sorted(d for d in os.listdir(tmp_path) if d.startswith("state."))
-    assert dumps == ["state.00000", "state.00001", "state.00002"]
+    assert dumps == ["state.00000", "state.00001"]
     U0 = load(tmp_path / dumps[0], "conserved", FIELDS)
-    U1 = load(tmp_path / dumps[1], "conserved", FIELDS)
     assert U0[0].shape == tuple(k + 4 for k in n) and U0[0].dtype == np.float64
-    o = Oracle(n, np.float32(ds), bcs, precision="f32", gamma=5.0 / 3.0, cfl=0.2, g=-0.1, gravity=gravity,
-               gravity_k_shift=0)
+    assert all(np.isfinite(u).all() for u in U0)
+    o = demo_oracle(demo)
     o.set_conserved([u.astype(np.float32) for u in U0])
     o.steps(1)
+    d1 = tmp_path / dumps[1]
+    U1 = load(d1, "conserved", FIELDS)
+    W1, G1, F1 = load(d1, "primitives", PRIMS), load(d1, "gradients", PRIMS), load(d1, "fluxes", FIELDS)
+    assert G1[0].shape == U0[0].shape + (3,)
     for q in range(5):
         np.testing.assert_array_equal(np.array(o.interior_view(o.cons[q])), U1[q][2:-2, 2:-2, 2:-2].astype(np.float32))
-    W1 = load(tmp_path / dumps[1], "primitives", ("rho", "vx", "vy", "vz", "pressure"))
-    V = np.float32(ds) ** 3
-    np.testing.assert_allclose(W1[0][2:-2, 2:-2, 2:-2], U1[0][2:-2, 2:-2, 2:-2] / V, rtol=1e-6)
+        np.testing.assert_array_equal(np.asarray(o.prim[q]), W1[q].astype(np.float32))
+        np.testing.assert_array_equal(np.asarray(o.grad[q]), G1[q].astype(np.float32))
+        np.testing.assert_array_equal(np.asarray(o.flux[q]), F1[q].astype(np.float32))
+
+
+def test_cli_restart_continues_the_run_exactly(tmp_path):
+    """--restart from a dump continues the loop (state, clock, dump numbering)
+    and ends where the uninterrupted run ends, bit for bit."""
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    args = ["--demo", "RAYLEIGH_TAYLOR", "--tout", "0.004", "--total-time", "1e9", "--max-steps", "9", "--quiet"]
+    out_a = run_cli(args + ["--out", str(a)], str(a))
+    dumps = sorted(d for d in os.listdir(a) if d.startswith("state."))
+    assert len(dumps) >= 3, dumps
+    out_b = run_cli(args + ["--out", str(b), "--restart", str(a / dumps[1])], str(b))
+    assert "Restarting simulation RAYLEIGH_TAYLOR" in out_b
+    assert out_a.split("Simulation done")[1].split("s!")[1] == out_b.split("Simulation done")[1].split("s!")[1]
+    assert sorted(d for d in os.listdir(b) if d.startswith("state.")) == dumps[2:]
+    last = dumps[-1]
+    for group, names in (("conserved", FIELDS), ("primitives", PRIMS), ("gradients", PRIMS), ("fluxes", FIELDS)):
+        for x, y in zip(load(a / last, group, names), load(b / last, group, names)):
+            np.testing.assert_array_equal(x, y)
+    assert open(a / last / "meta.txt").read() == open(b / last / "meta.txt").read()
+
+
+@pytest.mark.parametrize("flag, value", [("--precision", "f64"), ("--math", "quick"), ("--limiter", "superbee")])
+def test_cli_rejects_unknown_option_values(tmp_path, flag, value):
+    r = subprocess.run([CLI, "--demo", "SHOCK_TUBE", "--tout", "1", "--total-time", "1", "--max-steps", "1", flag,
+                        value], capture_output=True, text=True, timeout=60, cwd=str(tmp_path))
+    assert r.returncode == 2 and "unknown" in r.stderr
 
 
 def test_cli_unknown_demo_lists_demos(tmp_path):

@@ -266,3 +266,77 @@ def test_conducting_walls_match_oracle(precision):
     o.set_primitives(w)
     o.compute(1e30, 40)
     np.testing.assert_array_equal(U, np.stack([np.array(o.interior_view(a)) for a in o.cons]))
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_conducting_wall_heat_from_extrapolated_state(axis):
+    """A gas whose extrapolated state differs from its start-of-step state at
+    the wall (uniform rho, p; tangential velocity growing linearly along
+    another axis): one FP64 step with a conducting lower wall is bit for bit
+    the oracle's, whose heat source reads the extrapolated ghost state
+    (pinned by a closed form in test_oracle_invariants.py)."""
+    from deepgreen import Simulation, Solver
+    n, ds = (8, 8, 8), 0.125
+    along = (axis + 1) % 3
+    shape = tuple(k + 4 for k in n)
+    vel = 1.0 * ((np.arange(shape[along]) - 2 + 0.5) * ds - 0.5)
+    sl = [None, None, None]
+    sl[along] = slice(None)
+    w = [np.ones(shape), np.zeros(shape), np.zeros(shape), np.zeros(shape), np.ones(shape)]
+    w[1 + along] = np.broadcast_to(vel[tuple(sl)], shape).copy()
+    bcs = [1] * 6
+    bcs[2 * axis], bcs[2 * axis + 1] = 3, 2
+    prm = dict(gamma=1.4, cfl=0.2, g=0.0)
+    _say(f"  conducting wall along {axis}, extrapolated state")
+    with Solver(Simulation(*n, ds, bcs, precision="fp64", **prm)) as s:
+        s.set_heat(300.0, 4.0, 0.004)
+        s.set_primitives(w)
+        dt = s.compute_step()
+        U = np.stack([np.array(s.interior(a)) for a in s.get_state()])
+    o = Oracle(n, ds, bcs, precision="f64", t_out=300.0, v_out=4.0, rs=0.004, **prm)
+    o.set_primitives(w)
+    o.calc_conserved()
+    dto = o.steps(1)[0]
+    assert dt == dto
+    np.testing.assert_array_equal(U, np.stack([np.array(o.interior_view(a)) for a in o.cons]))
+
+
+@pytest.mark.parametrize("precision, math, limiter, bcs, gravity", [
+    ("fp32", "exact", "none", [0, 0, 1, 1, 2, 2], True),
+    ("fp64", "exact", "minmod", [2, 1, 0, 0, 1, 2], False),
+    ("fp32", "fast", "vanleer", [1, 2, 2, 1, 0, 0], True),
+])
+def test_step_dump_matches_oracle_arrays(precision, math, limiter, bcs, gravity):
+    """dg_step_dump: the primitives of the step's start, the gradients and the
+    fluxes the reference grid holds after compute_step (ghost cells
+    included) equal the oracle's bit for bit, over several steps and two
+    tiles per axis; the dump does not change the step (fast math: the dumped
+    arrays are the exact arithmetic's on the fast run's state)."""
+    from deepgreen import Simulation, Solver
+    interior = (20, 18, 70)
+    w, ds = cases.smooth_random(interior, seed=9, amp_v=0.1, p_base=10.0)
+    prm = dict(gamma=5.0 / 3.0, cfl=0.2, g=-0.5)
+    sim = Simulation(*interior, ds, bcs, precision=precision, math=math, limiter=limiter, gravity=gravity, **prm)
+    _say(f"  step dump {precision}/{math}/{limiter} bcs={bcs}")
+    oprec = "f64" if precision == "fp64" else "f32"
+    with Solver(sim) as s, Solver(sim) as plain:
+        s.set_primitives(w)
+        plain.set_primitives(w)
+        for step in range(3):
+            U_start = s.get_state()
+            o = Oracle(interior, ds, bcs, precision=oprec, limiter=limiter, gravity=gravity, gravity_k_shift=0,
+                       **prm)
+            o.set_conserved(U_start)
+            dt, prim, grad, flux = s.compute_step_dump()
+            assert dt == plain.compute_step()
+            o.steps(1)
+            for q in range(5):
+                np.testing.assert_array_equal(prim[q], np.asarray(o.prim[q]))
+                np.testing.assert_array_equal(grad[q], np.asarray(o.grad[q]))
+                if math == "exact":
+                    np.testing.assert_array_equal(flux[q], np.asarray(o.flux[q]))
+                else:   # the fast run's dt (fast wave speeds) enters the extrapolated state
+                    fo = np.asarray(o.flux[q])
+                    np.testing.assert_allclose(flux[q], fo, rtol=1e-5, atol=1e-6 * np.abs(fo).max())
+            for a, b in zip(s.get_state(), plain.get_state()):
+                np.testing.assert_array_equal(a, b)

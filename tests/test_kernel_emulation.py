@@ -196,3 +196,34 @@ def test_emulated_conducting_walls_match_oracle(emu_lib, precision):
     o.set_primitives(w)
     o.compute(1e30, 17)
     np.testing.assert_array_equal(U, np.stack([np.array(o.interior_view(a)) for a in o.cons]))
+
+
+@pytest.mark.parametrize("precision, limiter, bcs, gravity", [
+    ("fp32", "none", [0, 0, 1, 1, 2, 2], True),
+    ("fp64", "minmod", [2, 1, 0, 0, 1, 2], False),
+])
+def test_emulated_step_dump_matches_oracle_arrays(emu_lib, precision, limiter, bcs, gravity):
+    """dg_step_dump hands back the oracle's primitives (start of the step),
+    gradients and fluxes -- ghost cells included -- after the same steps."""
+    import deepgreen
+    interior = (6, 5, 7)
+    w, ds = cases.smooth_random(interior, seed=8, amp_v=0.1, p_base=10.0)
+    prm = dict(gamma=5.0 / 3.0, cfl=0.2, g=-0.5)
+    sim = deepgreen.Simulation(*interior, ds, bcs, precision=precision, limiter=limiter, gravity=gravity, tile_y=4,
+                               **prm)
+    o = Oracle(interior, ds, bcs, precision="f64" if precision == "fp64" else "f32", limiter=limiter,
+               gravity=gravity, gravity_k_shift=0, **prm)
+    o.set_primitives(w)
+    o.calc_conserved()
+    with deepgreen.Solver(sim, lib=emu_lib) as s:
+        s.set_primitives(w)
+        for step in range(3):
+            dt, prim, grad, flux = s.compute_step_dump()
+            dto = o.steps(1)[0]
+            assert dt == float(dto)
+            for q in range(5):
+                np.testing.assert_array_equal(prim[q], np.asarray(o.prim[q]))
+                np.testing.assert_array_equal(grad[q], np.asarray(o.grad[q]))
+                np.testing.assert_array_equal(flux[q], np.asarray(o.flux[q]))
+        U = np.stack([np.array(s.interior(a)) for a in s.get_state()])
+    np.testing.assert_array_equal(U, np.stack([np.array(o.interior_view(a)) for a in o.cons]))

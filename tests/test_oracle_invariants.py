@@ -286,3 +286,55 @@ def test_conducting_wall_heat_closed_form(axis):
     assert np.abs(E1 - E0).max() > 0
     for a, b in zip(U0, o.cons[:4]):
         np.testing.assert_allclose(np.array(o.interior_view(b)), a, rtol=0, atol=1e-10 * rho * ds ** 3)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_conducting_wall_heat_reads_the_extrapolated_state(axis):
+    """The heat source takes t_in from the half-step extrapolated (xtr) state of
+    the wall ghost, as the CUDA flux kernel reads rho_xtr / p_xtr
+    (physics.hpp:437-442).  Closed form: uniform rho, p and a velocity
+    tangential to the wall that grows linearly along another axis (divergence
+    alpha), so that away from the other boundaries the Hancock state of the
+    first layer is rho (1 - alpha dt/2), p (1 - gamma alpha dt/2) exactly.
+    The energy of a run with the conducting wall minus that of a run with a
+    plain wall is dt ds^2 Q / V in the first layer and 0 elsewhere, with Q
+    from that t_in; the start-of-step t_in would give a different increment."""
+    n, ds = [8, 8, 8], 0.125
+    t_out, v_out, rs, gamma = 300.0, 4.0, 0.004, 1.4
+    along = (axis + 1) % 3              # the velocity's direction and the axis it varies along
+    shape = tuple(k + 4 for k in n)
+    coord = (np.arange(shape[along]) - 2 + 0.5) * ds
+    alpha = 1.0
+    vel = alpha * (coord - 0.5)
+    sl = [None, None, None]
+    sl[along] = slice(None)
+    w = [np.ones(shape), np.zeros(shape), np.zeros(shape), np.zeros(shape), np.ones(shape)]
+    w[1 + along] = np.broadcast_to(vel[tuple(sl)], shape).copy()
+    runs = {}
+    for bc_lo in (2, 3):
+        bcs = [1] * 6
+        bcs[2 * axis], bcs[2 * axis + 1] = bc_lo, 2
+        o = Oracle(tuple(n), ds, bcs, precision="f64", gamma=gamma, cfl=0.2, g=0.0, t_out=t_out, v_out=v_out,
+                   rs=rs)
+        o.set_primitives(w)
+        o.calc_conserved()
+        dt = o.steps(1)[0]
+        runs[bc_lo] = (dt, np.array(o.interior_view(o.cons[4])).copy())
+    (dt, e_wall), (dt2, e_heat) = runs[2], runs[3]
+    assert dt == dt2
+    diff = e_heat - e_wall
+    layer = [slice(None)] * 3
+    layer[axis] = 0
+    assert np.all(np.delete(diff, 0, axis=axis) == 0)
+    hc = 10.0 - v_out + 10.0 * np.sqrt(v_out)
+
+    def incr(t_in):
+        return dt * ds * ((t_out - t_in) * (ds * ds / (0.008 / 0.2 + 1.0 / hc)) / ds ** 3 * ds)
+
+    t_xtr = (1.0 - gamma * alpha * dt / 2) / (rs * (1.0 - alpha * dt / 2))
+    inner = [slice(None)] * 3
+    inner[along] = slice(1, -1)          # away from the outflow faces along `along`
+    got = diff[tuple(layer)][tuple(inner[:axis] + inner[axis + 1:])]
+    np.testing.assert_allclose(got, incr(t_xtr), rtol=1e-11)
+    t_start = 1.0 / rs                   # p / (rs rho) at the start of the step
+    assert abs(incr(t_start) - incr(t_xtr)) > 1e-3 * abs(incr(t_xtr))
