@@ -33,7 +33,7 @@ STATUS = {0: "ok", -1: "invalid argument", -2: "device error", -3: "allocation f
 
 # every entry point declared in include/deepgreen.h
 EXPORTS = ("dg_config_default", "dg_create", "dg_destroy", "dg_last_error", "dg_set_state",
-           "dg_get_state", "dg_step", "dg_step_dump", "dg_run", "dg_reset_clock", "dg_conservation_sums",
+           "dg_get_state", "dg_step", "dg_step_dump", "dg_step_noalloc", "dg_run", "dg_reset_clock", "dg_conservation_sums",
            "dg_next_dt", "dg_device_state", "dg_time_steps", "dg_kernel_info", "dg_device_count", "dg_set_heat",
            "dg_device_info",
            "dg_ensemble_create", "dg_ensemble_destroy", "dg_ensemble_last_error", "dg_ensemble_set_state",
@@ -92,6 +92,7 @@ def load_library(path=None):
     L.dg_set_state.argtypes = [vp, P(vp), i32]
     L.dg_get_state.argtypes = [vp, P(vp), i32]
     L.dg_step.argtypes = [vp, P(d)]
+    L.dg_step_noalloc.argtypes = [P(DgConfig), P(vp), P(vp), vp, P(d)]
     L.dg_step_dump.argtypes = [vp, P(d), P(vp), P(vp), P(vp)]
     L.dg_run.argtypes = [vp, i64, d, P(d), P(i64)]
     L.dg_reset_clock.argtypes = [vp]

@@ -121,6 +121,25 @@ int dg_step(dg_ctx *ctx, double *dt_used);
  * dg_destroy.  Fluxes do not include the conducting-wall heat source. */
 int dg_step_dump(dg_ctx *ctx, double *dt_used, void *const prim[5], void *const grad[5], void *const flux[5]);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): one dg_step on caller-owned device buffers that calls no
+ * allocating routine.  in[5] and out[5] are device arrays of the interior
+ * conserved state (C order, n[0] x n[1] x n[2], the layout dg_device_state
+ * exposes; out must not alias in).  dt is computed from `in` (as after
+ * dg_set_state), the advanced state is written to out, and, if dt_used is not
+ * NULL, the call synchronizes `stream` (a hipStream_t; NULL = the null stream)
+ * and returns the dt (with dt_nan_fallback = 0 a non-finite dt returns
+ * DG_ERR_NONFINITE before the step, as dg_step).  With dt_used NULL the call
+ * only enqueues kernels (capturable).  The partial results and scalars live in
+ * a static device array of the library, one per GPU, held until the process
+ * ends: grids whose step needs more than DG_NOALLOC_MAX_SLOTS workgroups (or
+ * conducting walls) are rejected with DG_ERR_INVALID, and calls that overlap
+ * on one GPU race on that array. */
+#define DG_NOALLOC_MAX_SLOTS 8192
+int dg_step_noalloc(const dg_config *cfg, const void *const in[5], void *const out[5], void *stream,
+                    double *dt_used);
+
 /* The loop of main.rs:51-67 without output: steps while
  * !(t > t_end || steps >= max_steps), accumulating t += dt in the precision's
  * time type (float for DG_FP32, as the reference).  Runs from a captured HIP

@@ -198,6 +198,35 @@ def test_emulated_conducting_walls_match_oracle(emu_lib, precision):
     np.testing.assert_array_equal(U, np.stack([np.array(o.interior_view(a)) for a in o.cons]))
 
 
+def test_emulated_noalloc_step_matches_dg_step(emu_lib):
+    """dg_step_noalloc (static scratch array, caller-owned buffers) gives the
+    state and dt of dg_step bit for bit, on two calls in a row, the second
+    on a smaller grid."""
+    import ctypes
+
+    import deepgreen
+    for interior, bcs, precision, limiter in (((7, 6, 9), [0, 0, 1, 1, 2, 2], "fp32", "none"),
+                                              ((5, 5, 6), [1, 1, 0, 0, 2, 2], "fp64", "minmod")):
+        w, ds = cases.smooth_random(interior, seed=4)
+        sim = deepgreen.Simulation(*interior, ds, bcs, precision=precision, limiter=limiter, gamma=1.4, cfl=0.4,
+                                   g=0.0, tile_y=4)
+        with deepgreen.Solver(sim, lib=emu_lib) as ref, deepgreen.Solver(sim, lib=emu_lib) as a, \
+                deepgreen.Solver(sim, lib=emu_lib) as b:
+            for s in (ref, a, b):
+                s.set_primitives(w)
+            dt_ref = ref.compute_step()
+            U_ref = ref.get_state()
+            cfg = sim.config(emu_lib)
+            ins = (ctypes.c_void_p * 5)(*a.device_pointers())
+            outs = (ctypes.c_void_p * 5)(*b.device_pointers())
+            dt = ctypes.c_double()
+            assert emu_lib.dg_step_noalloc(ctypes.byref(cfg), ins, outs, None, ctypes.byref(dt)) == 0
+            assert dt.value == dt_ref
+            U = b.get_state()
+            for f in range(5):
+                np.testing.assert_array_equal(b.interior(U[f]), ref.interior(U_ref[f]))
+
+
 @pytest.mark.parametrize("precision, limiter, bcs, gravity", [
     ("fp32", "none", [0, 0, 1, 1, 2, 2], True),
     ("fp64", "minmod", [2, 1, 0, 0, 1, 2], False),
