@@ -243,7 +243,7 @@ def bench_learned(args):
     gen_s = time.perf_counter() - t0
     m.fit_normalization()
     t0 = time.perf_counter()
-    losses = m.train(args.train_steps, batch=8, crop=32, lr=1e-3, seed=args.seed)
+    losses = m.train(args.train_steps, batch=8, crop=32, lr=1e-3, seed=args.seed, log=log)
     train_s = time.perf_counter() - t0
     log(f"[learned] {m.samples} samples in {gen_s:.1f} s; {args.train_steps} Adam steps in {train_s:.1f} s, "
         f"loss {losses[:5].mean():.4g} -> {losses[-50:].mean():.4g}")

@@ -24,6 +24,9 @@ namespace {
 
 const int kGraphSteps = 16;  // even: the buffer parity is unchanged by a replay
 
+// why the calling thread's last dg_create failed (dg_last_error(NULL))
+thread_local std::string t_create_err;
+
 int fail(dg_ctx *c, int code, const std::string &msg)
 {
     if (c) c->err = msg;
@@ -122,7 +125,11 @@ void dg_config_default(dg_config *cfg)
     cfg->dt_nan_fallback = 1;
 }
 
-const char *dg_last_error(const dg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *dg_last_error(const dg_ctx *ctx)
+{
+    if (ctx) return ctx->err.c_str();
+    return t_create_err.empty() ? "null context" : t_create_err.c_str();
+}
 
 int dg_device_count(int *count)
 {
@@ -159,7 +166,13 @@ int dg_device_info(int device, char *arch, int len, int *compute_units)
     return DG_OK;
 }
 
-int dg_create(const dg_config *cfg, dg_ctx **out) { return dg::create_ctx(cfg, nullptr, out); }
+int dg_create(const dg_config *cfg, dg_ctx **out)
+{
+    std::string why;
+    const int rc = dg::create_ctx(cfg, nullptr, out, &why);
+    t_create_err = rc ? why : std::string();
+    return rc;
+}
 
 } // extern "C"
 
@@ -214,24 +227,29 @@ int set_tiling(dg_ctx *c, int tile_y, int chunk_x)
 
 // Validates a configuration and derives the kernel arguments (grid,
 // boundaries, constants, default tiling) from it.
-int make_args(const dg_config *cfg, StepArgs *out, int *tile_y_out, int *heat_mask_out)
+int make_args(const dg_config *cfg, StepArgs *out, int *tile_y_out, int *heat_mask_out, std::string *why)
 {
-    if (!cfg || !out) return DG_ERR_INVALID;
-    if (cfg->nghosts != 2) return DG_ERR_INVALID;
+    auto bad = [why](const char *msg) {
+        if (why) *why = msg;
+        return DG_ERR_INVALID;
+    };
+    if (!cfg || !out) return bad("null argument");
+    if (cfg->nghosts != 2) return bad("nghosts must be 2");
     for (int d = 0; d < 3; ++d)
-        if (cfg->n[d] < 2 || cfg->n[d] > (1 << 20)) return DG_ERR_INVALID;
+        if (cfg->n[d] < 2 || cfg->n[d] > (1 << 20)) return bad("interior size out of range [2, 2^20]");
     for (int i = 0; i < 6; ++i)
-        if (cfg->bc[i] < 0 || cfg->bc[i] > 3) return DG_ERR_INVALID;
+        if (cfg->bc[i] < 0 || cfg->bc[i] > 3) return bad("unknown boundary type");
     for (int d = 0; d < 3; ++d)  // periodic faces pair up (grid.rs:55-63)
-        if ((cfg->bc[2 * d] == DG_PERIODIC) != (cfg->bc[2 * d + 1] == DG_PERIODIC)) return DG_ERR_INVALID;
-    if (!(cfg->ds > 0) || !(cfg->gamma > 1) || !(cfg->cfl > 0)) return DG_ERR_INVALID;
-    if (cfg->precision < 0 || cfg->precision > 2 || cfg->math < 0 || cfg->math > 1) return DG_ERR_INVALID;
-    if (cfg->limiter < 0 || cfg->limiter > 2) return DG_ERR_INVALID;
-    if (cfg->precision == DG_FP64 && cfg->math == DG_MATH_FAST) return DG_ERR_INVALID;
+        if ((cfg->bc[2 * d] == DG_PERIODIC) != (cfg->bc[2 * d + 1] == DG_PERIODIC))
+            return bad("a periodic face needs a periodic opposite face");
+    if (!(cfg->ds > 0) || !(cfg->gamma > 1) || !(cfg->cfl > 0)) return bad("ds, cfl must be > 0 and gamma > 1");
+    if (cfg->precision < 0 || cfg->precision > 2 || cfg->math < 0 || cfg->math > 1)
+        return bad("unknown precision or math");
+    if (cfg->limiter < 0 || cfg->limiter > 2) return bad("unknown limiter");
     const size_t n3 = (size_t)cfg->n[0] * cfg->n[1] * cfg->n[2];
     // the kernels address a field with 32-bit byte offsets
     const size_t el_bytes = cfg->precision == DG_FP64 ? sizeof(double) : sizeof(float);
-    if (n3 * el_bytes >= ((size_t)1 << 32)) return DG_ERR_INVALID;
+    if (n3 * el_bytes >= ((size_t)1 << 32)) return bad("a field of this grid exceeds 4 GiB");
     const int prec = cfg->precision, fast = cfg->math == DG_MATH_FAST;
     // default tile height: 8 rows (two workgroups per CU); FP32 exact math
     // keeps one 16-row workgroup per CU (its register footprint allows no
@@ -260,7 +278,7 @@ int make_args(const dg_config *cfg, StepArgs *out, int *tile_y_out, int *heat_ma
     }
     a.gamma = cfg->gamma;
     a.g = cfg->g;
-    if (tile_args(a, prec, ty, cfg->chunk_x, nullptr) != DG_OK) return DG_ERR_INVALID;
+    if (tile_args(a, prec, ty, cfg->chunk_x, nullptr) != DG_OK) return bad("invalid tile_y or chunk_x");
     *out = a;
     if (tile_y_out) *tile_y_out = ty;
     if (heat_mask_out) *heat_mask_out = heat_mask;
@@ -278,8 +296,10 @@ int reserve_partials(dg_ctx *c)
         c->partials = nullptr;
         c->partials_cap = 0;
     }
-    if (hipMalloc((void **)&c->partials, (size_t)need * dg::PARTIAL_STRIDE * sizeof(double)) != hipSuccess)
+    if (hipMalloc((void **)&c->partials, (size_t)need * dg::PARTIAL_STRIDE * sizeof(double)) != hipSuccess) {
+        c->partials = nullptr;
         return DG_ERR_ALLOC;
+    }
     c->partials_cap = need;
     return DG_OK;
 }
@@ -326,16 +346,22 @@ int reduce_current(dg_ctx *c, int mode)
 }
 
 // Contexts of an ensemble share the ensemble's stream (shared_stream != null).
-int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out)
+int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out, std::string *why)
 {
-    if (!cfg || !out) return DG_ERR_INVALID;
+    auto failed = [why](int code, const std::string &msg) {
+        if (why) *why = msg;
+        return code;
+    };
+    if (!cfg || !out) return failed(DG_ERR_INVALID, "null argument");
     *out = nullptr;
     dg::StepArgs args{};
     int ty = 8, heat_mask = 0;
-    if (make_args(cfg, &args, &ty, &heat_mask) != DG_OK) return DG_ERR_INVALID;
+    std::string reason;
+    if (make_args(cfg, &args, &ty, &heat_mask, &reason) != DG_OK)
+        return failed(DG_ERR_INVALID, "invalid configuration: " + reason);
 
     dg_ctx *c = new (std::nothrow) dg_ctx();
-    if (!c) return DG_ERR_ALLOC;
+    if (!c) return failed(DG_ERR_ALLOC, "host allocation of the context failed");
     static std::atomic<uint64_t> next_uid{1};
     c->uid = next_uid.fetch_add(1);
     c->cfg = *cfg;
@@ -346,29 +372,45 @@ int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out)
     c->args = args;
     c->heat_mask = heat_mask;
     c->nparts_reduce = dg::reduce_blocks(args);
-    if (set_tiling(c, ty, cfg->chunk_x) != DG_OK) { delete c; return DG_ERR_INVALID; }
+    if (set_tiling(c, ty, cfg->chunk_x) != DG_OK) { delete c; return failed(DG_ERR_INVALID, "invalid tiling"); }
     const size_t n3 = c->n3;
 
     hipError_t e = hipSetDevice(cfg->device);
-    if (e != hipSuccess) { delete c; return DG_ERR_DEVICE; }
+    if (e != hipSuccess) { delete c; return failed(DG_ERR_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e)); }
     if (shared_stream) {
         c->stream = shared_stream;
         c->owns_stream = false;
     } else {
         e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-        if (e != hipSuccess) { delete c; return DG_ERR_DEVICE; }
+        if (e != hipSuccess) {
+            delete c;
+            return failed(DG_ERR_DEVICE, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+        }
     }
     int rc = DG_OK;
+    std::string what;
     for (int b = 0; b < 2 && rc == DG_OK; ++b)
         for (int f = 0; f < 5 && rc == DG_OK; ++f)
-            if (hipMalloc(&c->buf[b][f], n3 * c->el) != hipSuccess) rc = DG_ERR_ALLOC;
-    if (rc == DG_OK) rc = reserve_partials(c);
-    if (rc == DG_OK && hipMalloc((void **)&c->sc, sizeof(dg::Scalars)) != hipSuccess) rc = DG_ERR_ALLOC;
-    if (rc == DG_OK && hipHostMalloc((void **)&c->sc_host, sizeof(dg::Scalars), hipHostMallocDefault) != hipSuccess)
+            if (hipMalloc(&c->buf[b][f], n3 * c->el) != hipSuccess) {
+                c->buf[b][f] = nullptr;
+                rc = DG_ERR_ALLOC;
+                what = "device allocation of the state (10 arrays of " + std::to_string(n3 * c->el) + " bytes) failed";
+            }
+    if (rc == DG_OK && (rc = reserve_partials(c)) != DG_OK) what = "device allocation of the partial results failed";
+    if (rc == DG_OK && hipMalloc((void **)&c->sc, sizeof(dg::Scalars)) != hipSuccess) {
+        c->sc = nullptr;
         rc = DG_ERR_ALLOC;
+        what = "device allocation of the scalars failed";
+    }
+    if (rc == DG_OK && hipHostMalloc((void **)&c->sc_host, sizeof(dg::Scalars), hipHostMallocDefault) != hipSuccess) {
+        c->sc_host = nullptr;
+        rc = DG_ERR_ALLOC;
+        what = "pinned host allocation of the scalars failed";
+    }
     if (rc != DG_OK) {
-        dg_destroy(c);
-        return rc;
+        (void)hipGetLastError();   // the failed allocation's error is reported here, not by a later call
+        dg_destroy(c);             // releases whatever was allocated
+        return failed(rc, what);
     }
     std::memset(c->sc_host, 0, sizeof(dg::Scalars));
     c->sc_host->t_end = INFINITY;
@@ -376,7 +418,7 @@ int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out)
     c->sc_host->active = 1;
     if (hipMemcpy(c->sc, c->sc_host, sizeof(dg::Scalars), hipMemcpyHostToDevice) != hipSuccess) {
         dg_destroy(c);
-        return DG_ERR_DEVICE;
+        return failed(DG_ERR_DEVICE, "initializing the scalars");
     }
     *out = c;
     return DG_OK;

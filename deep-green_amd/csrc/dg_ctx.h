@@ -42,10 +42,11 @@ struct dg_ctx {
 namespace dg {
 // Validates a configuration and derives the kernel arguments (default tiling
 // included); tile_y_out / heat_mask_out may be null.
-int make_args(const dg_config *cfg, StepArgs *out, int *tile_y_out, int *heat_mask_out);
+int make_args(const dg_config *cfg, StepArgs *out, int *tile_y_out, int *heat_mask_out, std::string *why = nullptr);
 // Workgroup tiling of a grid (tile_y rows, chunk_x planes; 0 = automatic).
 int tile_args(StepArgs &a, int prec, int tile_y, int chunk_x, int *chunk_out);
-int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out);
+// why (may be null) receives the reason of a failure
+int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out, std::string *why = nullptr);
 int set_tiling(dg_ctx *c, int tile_y, int chunk_x);
 int reserve_partials(dg_ctx *c);
 // One step (step kernel, then dt and clock: finalize mode 1); flips c->cur.

@@ -274,6 +274,7 @@ int dg_ensemble_create(const dg_config *cfgs, int count, int grid, dg_ensemble *
             return DG_ERR_INVALID;
     }
     if (c0.precision == DG_FP64 && ty0 > 8) return DG_ERR_INVALID;
+    if (c0.precision == DG_FP64 && c0.math == DG_MATH_FAST) return DG_ERR_INVALID;   // FP32 fast math only
 
     dg_ensemble *e = new (std::nothrow) dg_ensemble();
     if (!e) return DG_ERR_ALLOC;

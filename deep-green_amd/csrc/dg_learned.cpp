@@ -103,13 +103,18 @@ int mfail(dg_cnn *m, int code, const std::string &msg)
         if ((expr) != 0) return mfail((m), DG_ERR_DEVICE, (what));         \
     } while (0)
 
-// workspace slot k of at least n bytes
+void drop_graphs(dg_cnn *m);
+
+// workspace slot k of at least n bytes.  Growing a slot frees its old
+// buffer, which captured learned steps may reference: they are dropped (and
+// captured again by the next learned run).
 void *ws(dg_cnn *m, int k, size_t n)
 {
     Buf &b = m->ws[k];
     if (b.n >= n && b.p) return b.p;
     if (b.p) {
         (void)hipStreamSynchronize(m->stream);
+        drop_graphs(m);
         (void)hipFree(b.p);
     }
     b.p = nullptr;

@@ -61,7 +61,24 @@ template <> __device__ __forceinline__ float dabs<float>(float x) { return fabsf
 template <> __device__ __forceinline__ double dabs<double>(double x) { return fabs(x); }
 template <typename T> __device__ __forceinline__ T drcp(T x);
 template <> __device__ __forceinline__ float drcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
-template <> __device__ __forceinline__ double drcp<double>(double x) { return 1.0 / x; }
+// FP64 fast math: the hardware reciprocal estimate refined by two Newton
+// steps (each squares the relative error), within an ulp or two of 1/x
+template <> __device__ __forceinline__ double drcp<double>(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+// reciprocal square root: hardware estimate (FP32); FP64 refined by two
+// Newton steps y <- y + y (1 - x y^2) / 2
+template <typename T> __device__ __forceinline__ T drsq(T x);
+template <> __device__ __forceinline__ float drsq<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template <> __device__ __forceinline__ double drsq<double>(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+    return fma(0.5 * y, fma(-x * y, y, 1.0), y);
+}
 template <typename T> __device__ __forceinline__ T dsqrt_fast(T x);
 template <> __device__ __forceinline__ float dsqrt_fast<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 template <> __device__ __forceinline__ double dsqrt_fast<double>(double x) { return sqrt(x); }
@@ -176,7 +193,8 @@ __device__ __forceinline__ T wave_den(const T w[5], T gamma)
         return dsqrt<T>(gamma * w[4] / w[0]) + dsqrt<T>(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
     } else {
         const T v2 = dfma<T>(w[3], w[3], dfma<T>(w[2], w[2], w[1] * w[1]));
-        return dsqrt_fast<T>(gamma * w[4] * drcp<T>(w[0])) + dsqrt_fast<T>(v2);
+        const T gp = gamma * w[4];
+        return dfma<T>(gp, drsq<T>(gp * w[0]), dsqrt_fast<T>(v2));
     }
 }
 
@@ -248,8 +266,10 @@ __device__ __forceinline__ void rusanov(const T l[5], const T r[5], T F[5], T ga
             f1 = dfma<T>(Sr, gh, f1);
             f4 = dfma<T>(T(0.5) * Sr * (vxl + vxr), gh, f4);
         }
-        const T al = dsqrt_fast<T>(gamma * pl * drcp<T>(rl)) + dabs<T>(vxl);
-        const T ar = dsqrt_fast<T>(gamma * pr * drcp<T>(rr)) + dabs<T>(vxr);
+        // sqrt(gamma p / rho) = gamma p / sqrt(gamma p rho): one reciprocal square root
+        const T gpl = gamma * pl, gpr = gamma * pr;
+        const T al = dfma<T>(gpl, drsq<T>(gpl * rl), dabs<T>(vxl));
+        const T ar = dfma<T>(gpr, drsq<T>(gpr * rr), dabs<T>(vxr));
         const T c = dmax<T>(al, ar);
         F[0] = dfma<T>(-c, rl - rr, f0);
         F[1] = dfma<T>(-c, mxl - mxr, f1);
@@ -892,8 +912,9 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
     block_partials<NT>(dmaxden, nanf, nosums, red, partial_out, 0, tid);
 }
 
-// Occupancy target: the FP32 fast kernels fit 128 VGPRs (4 waves per SIMD).
-template <typename T, bool FAST> constexpr int waves_per_eu() { return (FAST && sizeof(T) == 4) ? 4 : 1; }
+// Occupancy target: the FP32 fast kernels fit 128 VGPRs (4 waves per SIMD),
+// the FP64 fast kernels 256 (2 waves per SIMD).
+template <typename T, bool FAST> constexpr int waves_per_eu() { return FAST ? (sizeof(T) == 4 ? 4 : 2) : 1; }
 
 template <typename T, int TY, bool FAST, bool BULK, int LIM>
 __global__ void __launch_bounds__(TY * TZ) __attribute__((amdgpu_waves_per_eu(waves_per_eu<T, FAST>())))
@@ -1341,8 +1362,8 @@ int launch_step(int prec, int fast, int tile_y, const StepArgs &a, const void *c
 {
 #define DG_CASE(T, TYV, F, B) launch_step_t<T, TYV, F, B>(a, in, out, sc, partials, s)
     if (prec == 0) {
-        if (tile_y == 4) DG_CASE(double, 4, false, false);
-        else if (tile_y == 8) DG_CASE(double, 8, false, false);
+        if (tile_y == 4) { if (fast) DG_CASE(double, 4, true, false); else DG_CASE(double, 4, false, false); }
+        else if (tile_y == 8) { if (fast) DG_CASE(double, 8, true, false); else DG_CASE(double, 8, false, false); }
         else return -1;
     } else {
         const bool bulk = prec == 2;
@@ -1519,7 +1540,10 @@ int launch_heat(int prec, int fast, const StepArgs &a, const void *const uold[5]
     HeatArgs H{t_out, v_out, rs, mask};
     const long long cells = (long long)a.n[1] * a.n[2] + (long long)a.n[0] * a.n[2] + (long long)a.n[0] * a.n[1];
     const int blocks = (int)std::min<long long>((cells + 255) / 256, 2048);
-    if (prec == 0)
+    if (prec == 0 && fast)
+        hipLaunchKernelGGL((heat_kernel<double, true>), dim3(blocks), dim3(256), 0, s, a, cfields<double>(uold),
+                           mfields<double>(unew), sc, H);
+    else if (prec == 0)
         hipLaunchKernelGGL((heat_kernel<double, false>), dim3(blocks), dim3(256), 0, s, a, cfields<double>(uold),
                            mfields<double>(unew), sc, H);
     else if (fast)
@@ -1552,8 +1576,10 @@ int launch_reduce(int prec, int fast, const StepArgs &a, const void *const u[5],
                   hipStream_t s)
 {
     const dim3 g(nblocks), b(256);
-    if (prec == 0)
+    if (prec == 0 && !fast)
         hipLaunchKernelGGL((reduce_kernel<double, false, false>), g, b, 0, s, a, cfields<double>(u), partials);
+    else if (prec == 0)
+        hipLaunchKernelGGL((reduce_kernel<double, false, true>), g, b, 0, s, a, cfields<double>(u), partials);
     else if (prec == 1 && !fast)
         hipLaunchKernelGGL((reduce_kernel<float, false, false>), g, b, 0, s, a, cfields<float>(u), partials);
     else if (prec == 1)

@@ -144,11 +144,23 @@ class FluxCNN:
         """Gaussian noise (normalized units) added to the inputs while training."""
         self._check(self._L.dg_cnn_set_input_noise(self._h, float(std)), "dg_cnn_set_input_noise")
 
-    def train(self, steps, batch=8, crop=32, lr=1e-3, seed=0):
-        losses = np.empty(int(steps), np.float32)
-        self._check(self._L.dg_cnn_train(self._h, int(steps), int(batch), int(crop), float(lr), int(seed),
-                                         _fp(losses)), "dg_cnn_train")
-        return losses
+    def train(self, steps, batch=8, crop=32, lr=1e-3, seed=0, chunk=250, log=None):
+        """Adam on random crops of the samples, `chunk` steps per library call
+        (crops of chunk k drawn from seed + k); `log` gets a progress line after
+        every chunk.  Returns the per-step losses."""
+        out = []
+        done, k = 0, 0
+        while done < int(steps):
+            n = min(int(chunk), int(steps) - done)
+            losses = np.empty(n, np.float32)
+            self._check(self._L.dg_cnn_train(self._h, n, int(batch), int(crop), float(lr), int(seed) + k,
+                                             _fp(losses)), "dg_cnn_train")
+            out.append(losses)
+            done += n
+            k += 1
+            if log:
+                log(f"[learned] Adam step {done}/{int(steps)}: loss {float(losses[-min(n, 50):].mean()):.4g}")
+        return np.concatenate(out)
 
     # learned solver -----------------------------------------------------------
     def step(self, solver, dt=0.0):
@@ -244,11 +256,15 @@ def make_training_set(model, seeds, n_fine=256, t_end=0.03, every=8, log=None):
         coarse = Solver(_sim(n_fine // 2, 2 * ds))
         fine.set_primitives(w)
         del w
+        if log:
+            log(f"[learned] seed {seed}: {n_fine}^3 initial condition ready")
         t = 0.0
         while t < t_end:
             model.add_sample(fine, coarse)
             fine.run(max(every - 2, 0))
             t = clock(fine)[0]
+            if log and model.samples % 4 == 0:
+                log(f"[learned] seed {seed}: {model.samples} samples, t = {t:.4f}")
         if log:
             log(f"[learned] seed {seed}: {model.samples} samples, t = {t:.4f}")
         fine.close()

@@ -87,6 +87,9 @@ template <typename T> inline T __shfl_up(T v, unsigned d) { return emu::exch<T>(
 template <typename T> inline T __shfl_xor(T v, int m) { return emu::exch<T>(v, m, 2); }
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 inline float __builtin_amdgcn_sqrtf(float x) { return std::sqrt(x); }
+inline float __builtin_amdgcn_rsqf(float x) { return 1.0f / std::sqrt(x); }
+inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
+inline double __builtin_amdgcn_rsq(double x) { return 1.0 / std::sqrt(x); }
 // DPP wave_shl:1 (0x130: lane i <- lane i+1) and wave_shr:1 (0x138: lane i <- lane i-1);
 // lanes without a source keep `old`.
 inline int __builtin_amdgcn_update_dpp(int old, int v, int ctrl, int, int, bool)

@@ -269,3 +269,31 @@ def test_learned_run_equals_single_learned_steps(emu_lib):
     np.testing.assert_array_equal(interior(a.get_state()), interior(b.get_state()))
     assert a.run(0) == b.run(0)
     a.close(), b.close()
+
+
+def test_learned_run_after_workspace_growth(emu_lib):
+    """A learned run, then a training batch whose crops need larger
+    workspaces than the run's (the slots the captured steps use are freed and
+    reallocated), then a learned run again: equal to the same runs of a model
+    that never trained in between (the captured steps are rebuilt, not replayed
+    on freed buffers)."""
+    from deepgreen.learned import FluxCNN
+    m = model(emu_lib, seed=15)
+    p, norm = m.get_params(), m.get_normalization()
+    a = periodic_solver(emu_lib, 5, 6)
+    m.run(a, 16)
+    rng = np.random.default_rng(0)
+    L = 8
+    x = rng.normal(size=(2, L + 7, L + 7, L + 7, 5)).astype(np.float32)
+    t = rng.normal(size=(2, L, L, L, 5)).astype(np.float32)
+    m.train_batch(x, t)                      # grows the input, activation and output slots
+    m.run(a, 16)
+    got = interior(a.get_state())
+    a.close()
+    m2 = FluxCNN(seed=16, lib=emu_lib)
+    m2.set_params(p)
+    m2.set_normalization(norm)
+    b = periodic_solver(emu_lib, 5, 6)
+    m2.run(b, 32)
+    np.testing.assert_array_equal(got, interior(b.get_state()))
+    b.close()

@@ -92,7 +92,12 @@ def test_cli_restart_continues_the_run_exactly(tmp_path):
     a, b = tmp_path / "a", tmp_path / "b"
     a.mkdir()
     b.mkdir()
-    args = ["--demo", "RAYLEIGH_TAYLOR", "--tout", "0.004", "--total-time", "1e9", "--max-steps", "9", "--quiet"]
+    probe = run_cli(["--demo", "RAYLEIGH_TAYLOR", "--tout", "1", "--total-time", "1e9", "--max-steps", "2",
+                     "--no-dump"], str(a))
+    dt = float(probe.split("Computing step 1 and time ")[1].split(" with dt ")[1].split()[0].rstrip("."))
+    # a dump about every third step
+    args = ["--demo", "RAYLEIGH_TAYLOR", "--tout", repr(2.5 * dt), "--total-time", "1e9", "--max-steps", "11",
+            "--quiet"]
     out_a = run_cli(args + ["--out", str(a)], str(a))
     dumps = sorted(d for d in os.listdir(a) if d.startswith("state."))
     assert len(dumps) >= 3, dumps

@@ -201,3 +201,49 @@ def test_samples_training_and_learned_run():
     s1 = coarse.conservation_sums()
     np.testing.assert_allclose(s1[0], s0[0], rtol=2e-6)
     assert np.isfinite(s1).all()
+
+
+def test_learned_workload_128():
+    """The learned configuration at its size: the 128^3 periodic grid of the
+    learned workload's initial condition (8 density modes, Gaussian pressure
+    blasts), FP32 fast math, minmod, with a fixed-seed network (a 10x output
+    layer, so the correction is not negligible).  Over 24 learned steps the
+    total mass is conserved to rounding (the correction telescopes), the
+    state stays finite, and the per-step fallback count (cells keeping the
+    plain flux) is reported; with a zero output layer the learned run equals
+    the plain run bit for bit."""
+    from deepgreen import Solver
+    from deepgreen.learned import FluxCNN, _sim, join_params
+    from deepgreen.setups import learned_ic
+    n = 128
+    w, ds = learned_ic(n, 7)
+    _say(f"  learned {n}^3 workload IC ready")
+    m = model(seed=3)
+    m.set_normalization(np.array([1.0, 0, 0, 0, 2.0, 0.2, 0.1, 0.1, 0.1, 2.0] + [0.1, 0.03, 0.03, 0.03, 0.3],
+                                 np.float32))
+    with Solver(_sim(n, ds)) as s, Solver(_sim(n, ds)) as plain, Solver(_sim(n, ds)) as lz:
+        for x in (s, plain, lz):
+            x.set_primitives(w)
+        M0 = s.conservation_sums()[0]
+        flagged = []
+        for _ in range(24):
+            m.step(s)
+            flagged.append(m.flagged())
+        sums = s.conservation_sums()
+        _say(f"  mass drift {(sums[0] - M0) / M0:.3e}, fallback cells per step {flagged}")
+        assert np.isfinite(sums).all()
+        assert all(isinstance(f, int) and 0 <= f < n ** 3 for f in flagged)
+        assert abs(sums[0] - M0) <= 1e-8 * M0
+        U = interior(s.get_state())
+        assert np.isfinite(U).all()
+        # zero correction: the learned run is the plain run
+        b = ref.split(m.get_params())
+        b["W3"] = np.zeros_like(b["W3"])
+        b["b3"] = np.zeros_like(b["b3"])
+        mz = FluxCNN(seed=3)
+        mz.set_params(join_params(b))
+        mz.set_normalization(m.get_normalization())
+        mz.run(lz, 24)
+        plain.run(24)
+        np.testing.assert_array_equal(interior(lz.get_state()), interior(plain.get_state()))
+        assert lz.run(0) == plain.run(0)

@@ -256,3 +256,16 @@ def test_emulated_step_dump_matches_oracle_arrays(emu_lib, precision, limiter, b
                 np.testing.assert_array_equal(flux[q], np.asarray(o.flux[q]))
         U = np.stack([np.array(s.interior(a)) for a in s.get_state()])
     np.testing.assert_array_equal(U, np.stack([np.array(o.interior_view(a)) for a in o.cons]))
+
+
+@pytest.mark.parametrize("limiter, bcs", [("none", [1, 1, 0, 0, 2, 2]), ("minmod", [0, 0, 2, 1, 0, 0])])
+def test_emulated_fp64_fast_close_to_oracle(emu_lib, limiter, bcs):
+    """FP64 fast math (reassociated, refined reciprocals) stays within 1e-13
+    relative L1 per field of the FP64 oracle over a few steps."""
+    interior = (7, 6, 9)
+    w, ds = cases.smooth_random(interior, seed=12)
+    U, dts, Uo, dto = run_both(emu_lib, interior, w, ds, bcs, 4, "fp64", limiter=limiter, math="fast")
+    rel = np.abs(U - Uo).reshape(5, -1).sum(1) / np.abs(Uo).reshape(5, -1).sum(1)
+    assert np.all(rel <= 1e-13), rel
+    np.testing.assert_allclose(dts, dto, rtol=1e-14)
+    assert not np.array_equal(U, Uo)   # the fast arithmetic is not the reference's order

@@ -42,7 +42,6 @@ def test_config_default_is_reference_constants():
     lambda c: setattr(c, "gamma", 1.0),
     lambda c: setattr(c, "precision", 5),
     lambda c: setattr(c, "limiter", 9),
-    lambda c: (setattr(c, "precision", 0), setattr(c, "math", 1)),
     lambda c: setattr(c, "tile_y", 5),
 ])
 def test_invalid_configs_are_rejected_before_device_use(mutate):
@@ -57,8 +56,32 @@ def test_invalid_configs_are_rejected_before_device_use(mutate):
 
 
 def test_null_arguments_return_errors():
+    import threading
+
     import deepgreen
     L = deepgreen.load_library()
+    seen = []
+    t = threading.Thread(target=lambda: seen.append(L.dg_last_error(None)))
+    t.start()
+    t.join()
+    assert seen == [b"null context"]          # no failed dg_create on that thread
     assert L.dg_create(None, None) == -1
+    assert L.dg_last_error(None) == b"null argument"
     assert L.dg_step(None, None) == -1
-    assert L.dg_last_error(None) == b"null context"
+
+
+def test_failed_create_explains_itself():
+    """dg_last_error(NULL) gives the reason of the calling thread's last failed
+    dg_create (no context exists to hold it)."""
+    import deepgreen
+    L = deepgreen.load_library()
+    c = deepgreen.DgConfig()
+    L.dg_config_default(ctypes.byref(c))
+    c.bc[0] = 1                                   # x- outflow against a periodic x+
+    h = ctypes.c_void_p()
+    assert L.dg_create(ctypes.byref(c), ctypes.byref(h)) == -1 and not h.value
+    assert L.dg_last_error(None) == b"invalid configuration: a periodic face needs a periodic opposite face"
+    c.bc[0] = 0
+    c.n[0] = c.n[1] = c.n[2] = 2048               # 32 GiB per field: beyond the 32-bit offsets
+    assert L.dg_create(ctypes.byref(c), ctypes.byref(h)) == -1
+    assert b"exceeds 4 GiB" in L.dg_last_error(None)
