@@ -239,6 +239,7 @@ def bench_learned(args):
     m = FluxCNN(seed=args.seed)
     t0 = time.perf_counter()
     make_training_set(m, seeds=range(1, 1 + args.train_ics), n_fine=2 * args.n_learned, t_end=args.t_end,
+                      trajectory=args.train_mode == "trajectory",
                       every=args.sample_every, log=log)
     gen_s = time.perf_counter() - t0
     m.fit_normalization()
@@ -270,6 +271,7 @@ def bench_learned(args):
         "config": {"workload": f"learned{n}_fp32", "error_matched": matched,
                    "l1_rho": e, "runs": r, "T": args.t_end, "train_ics": args.train_ics,
                    "samples": m.samples, "train_steps": args.train_steps, "batch": 8, "crop": 32, "lr": 1e-3,
+                   "train_mode": args.train_mode,
                    "loss_first": float(losses[:5].mean()), "loss_last": float(losses[-50:].mean()),
                    "data_gen_s": round(gen_s, 2), "train_s": round(train_s, 2),
                    "mass_learned_vs_plain": res["mass_learned_vs_plain"], "warmup_steps": args.learned_warmup,
@@ -348,6 +350,9 @@ def main():
     ap.add_argument("--train-ics", type=int, default=4)
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--sample-every", type=int, default=8)
+    ap.add_argument("--train-mode", default="reset", choices=["reset", "trajectory"],
+                    help="learned workload: samples of restricted fine states (reset) or of the coarse grid's own "
+                         "plain trajectory (trajectory)")
     ap.add_argument("--t-end", type=float, default=0.03)
     ap.add_argument("--learned-warmup", type=int, default=0, help="plain steps before the correction starts")
     ap.add_argument("--n", type=int, default=512)

@@ -637,7 +637,9 @@ int dg_cnn_adam_step(dg_cnn *m, const float *grads, float lr, float beta1, float
     return DG_OK;
 }
 
-int dg_cnn_add_sample(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse)
+int dg_cnn_add_sample(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse) { return dg_cnn_add_sample_mode(m, fine, coarse, 0, 1); }
+
+int dg_cnn_add_sample_mode(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse, int keep_coarse, int store)
 {
     if (!m || !fine || !coarse) return DG_ERR_INVALID;
     if (!periodic_fp32(fine) || !periodic_fp32(coarse)) return mfail(m, DG_ERR_INVALID, "FP32 periodic grids expected");
@@ -646,6 +648,7 @@ int dg_cnn_add_sample(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse)
     if (std::fabs(coarse->cfg.ds - 2 * fine->cfg.ds) > 1e-12 * coarse->cfg.ds)
         return mfail(m, DG_ERR_INVALID, "coarse ds must be 2x fine ds");
     if (!fine->have_state) return mfail(m, DG_ERR_STATE, "fine grid has no state");
+    if (keep_coarse && !coarse->have_state) return mfail(m, DG_ERR_STATE, "coarse grid has no state");
     if (!m->sU.empty())
         for (int d = 0; d < 3; ++d)
             if (m->sn[d] != coarse->args.n[d]) return mfail(m, DG_ERR_INVALID, "samples must share the coarse grid");
@@ -653,20 +656,25 @@ int dg_cnn_add_sample(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse)
     MHIP(m, hipStreamSynchronize(fine->stream));
     MHIP(m, hipStreamSynchronize(coarse->stream));
     const size_t n3 = coarse->n3;
-    // U^n = R(fine), with its dt
-    MCHK(m, dg::launch_restrict(fine->args, fine->buf[fine->cur], coarse->args, coarse->buf[coarse->cur],
-                                coarse->stream), "restrict");
-    if (dg::reduce_current(coarse, 0)) return mfail(m, DG_ERR_DEVICE, coarse->err);
-    coarse->have_state = true;
-    std::vector<float *> u(5), r(5);
-    for (int q = 0; q < 5; ++q) {
-        if (hipMalloc((void **)&u[q], n3 * 4) != hipSuccess || hipMalloc((void **)&r[q], n3 * 4) != hipSuccess) {
-            for (float *f : u) if (f) (void)hipFree(f);
-            for (float *f : r) if (f) (void)hipFree(f);
-            return mfail(m, DG_ERR_ALLOC, "sample storage");
-        }
-        MHIP(m, hipMemcpyAsync(u[q], coarse->buf[coarse->cur][q], n3 * 4, hipMemcpyDeviceToDevice, coarse->stream));
+    if (!keep_coarse) {
+        // U^n = R(fine), with its dt
+        MCHK(m, dg::launch_restrict(fine->args, fine->buf[fine->cur], coarse->args, coarse->buf[coarse->cur],
+                                    coarse->stream), "restrict");
+        if (dg::reduce_current(coarse, 0)) return mfail(m, DG_ERR_DEVICE, coarse->err);
+        coarse->have_state = true;
     }
+    // otherwise U^n is the coarse grid's own state (its trajectory), dt already on the device
+    std::vector<float *> u(5, nullptr), r(5, nullptr);
+    if (store)
+        for (int q = 0; q < 5; ++q) {
+            if (hipMalloc((void **)&u[q], n3 * 4) != hipSuccess || hipMalloc((void **)&r[q], n3 * 4) != hipSuccess) {
+                for (float *f : u) if (f) (void)hipFree(f);
+                for (float *f : r) if (f) (void)hipFree(f);
+                return mfail(m, DG_ERR_ALLOC, "sample storage");
+            }
+            MHIP(m, hipMemcpyAsync(u[q], coarse->buf[coarse->cur][q], n3 * 4, hipMemcpyDeviceToDevice,
+                                   coarse->stream));
+        }
     dg::Scalars sf, scs;
     MHIP(m, hipMemcpyAsync(&scs, coarse->sc, sizeof(scs), hipMemcpyDeviceToHost, coarse->stream));
     MHIP(m, hipMemcpyAsync(&sf, fine->sc, sizeof(sf), hipMemcpyDeviceToHost, fine->stream));
@@ -686,6 +694,12 @@ int dg_cnn_add_sample(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse)
         m->tmp = nullptr;
         MHIP(m, hipMalloc((void **)&m->tmp, 5 * n3 * 4));
         m->tmp_cells = n3;
+    }
+    if (!store) {
+        // both grids advanced by dt, nothing kept
+        MHIP(m, hipStreamSynchronize(fine->stream));
+        MHIP(m, hipStreamSynchronize(coarse->stream));
+        return DG_OK;
     }
     void *t[5];
     for (int q = 0; q < 5; ++q) t[q] = m->tmp + q * n3;

@@ -61,22 +61,20 @@ template <> __device__ __forceinline__ float dabs<float>(float x) { return fabsf
 template <> __device__ __forceinline__ double dabs<double>(double x) { return fabs(x); }
 template <typename T> __device__ __forceinline__ T drcp(T x);
 template <> __device__ __forceinline__ float drcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
-// FP64 fast math: the hardware reciprocal estimate refined by two Newton
-// steps (each squares the relative error), within an ulp or two of 1/x
+// FP64 fast math: the hardware reciprocal estimate refined by one Newton
+// step (which squares its relative error)
 template <> __device__ __forceinline__ double drcp<double>(double x)
 {
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(fma(-x, r, 1.0), r, r);
+    const double r = __builtin_amdgcn_rcp(x);
     return fma(fma(-x, r, 1.0), r, r);
 }
-// reciprocal square root: hardware estimate (FP32); FP64 refined by two
-// Newton steps y <- y + y (1 - x y^2) / 2
+// reciprocal square root: hardware estimate (FP32); FP64 refined by one
+// Newton step y <- y + y (1 - x y^2) / 2
 template <typename T> __device__ __forceinline__ T drsq(T x);
 template <> __device__ __forceinline__ float drsq<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 template <> __device__ __forceinline__ double drsq<double>(double x)
 {
-    double y = __builtin_amdgcn_rsq(x);
-    y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+    const double y = __builtin_amdgcn_rsq(x);
     return fma(0.5 * y, fma(-x * y, y, 1.0), y);
 }
 template <typename T> __device__ __forceinline__ T dsqrt_fast(T x);
@@ -194,7 +192,7 @@ __device__ __forceinline__ T wave_den(const T w[5], T gamma)
     } else {
         const T v2 = dfma<T>(w[3], w[3], dfma<T>(w[2], w[2], w[1] * w[1]));
         const T gp = gamma * w[4];
-        return dfma<T>(gp, drsq<T>(gp * w[0]), dsqrt_fast<T>(v2));
+        return dfma<T>(dabs<T>(gp), drsq<T>(gp * w[0]), dsqrt_fast<T>(v2));   // sqrt(gamma p / rho) + |v|
     }
 }
 
@@ -266,10 +264,13 @@ __device__ __forceinline__ void rusanov(const T l[5], const T r[5], T F[5], T ga
             f1 = dfma<T>(Sr, gh, f1);
             f4 = dfma<T>(T(0.5) * Sr * (vxl + vxr), gh, f4);
         }
-        // sqrt(gamma p / rho) = gamma p / sqrt(gamma p rho): one reciprocal square root
+        // sqrt(gamma p / rho) = |gamma p| / sqrt(gamma p rho): one reciprocal
+        // square root; the absolute value keeps the identity for the negative
+        // face pressures and densities the unlimited scheme produces at strong
+        // jumps (p / rho > 0), and a negative p / rho gives NaN as sqrt does
         const T gpl = gamma * pl, gpr = gamma * pr;
-        const T al = dfma<T>(gpl, drsq<T>(gpl * rl), dabs<T>(vxl));
-        const T ar = dfma<T>(gpr, drsq<T>(gpr * rr), dabs<T>(vxr));
+        const T al = dfma<T>(dabs<T>(gpl), drsq<T>(gpl * rl), dabs<T>(vxl));
+        const T ar = dfma<T>(dabs<T>(gpr), drsq<T>(gpr * rr), dabs<T>(vxr));
         const T c = dmax<T>(al, ar);
         F[0] = dfma<T>(-c, rl - rr, f0);
         F[1] = dfma<T>(-c, mxl - mxr, f1);

@@ -42,7 +42,7 @@ EXPORTS = ("dg_config_default", "dg_create", "dg_destroy", "dg_last_error", "dg_
            "dg_step_dt", "dg_restrict",
            "dg_cnn_create", "dg_cnn_destroy", "dg_cnn_last_error", "dg_cnn_param_count", "dg_cnn_get_params",
            "dg_cnn_set_params", "dg_cnn_get_normalization", "dg_cnn_set_normalization", "dg_cnn_forward",
-           "dg_cnn_backward", "dg_cnn_train_batch", "dg_cnn_adam_step", "dg_cnn_add_sample", "dg_cnn_sample_count", "dg_cnn_set_input_noise",
+           "dg_cnn_backward", "dg_cnn_train_batch", "dg_cnn_adam_step", "dg_cnn_add_sample", "dg_cnn_add_sample_mode", "dg_cnn_sample_count", "dg_cnn_set_input_noise",
            "dg_cnn_fit_normalization", "dg_cnn_train", "dg_learned_step", "dg_learned_run", "dg_learned_flagged",
            "dg_learned_correction", "dg_learned_step_given", "dg_learned_time")
 
@@ -136,6 +136,7 @@ def load_library(path=None):
     L.dg_cnn_train_batch.argtypes = [vp, i32, i32, fp, fp, P(d), fp]
     L.dg_cnn_adam_step.argtypes = [vp, fp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]
     L.dg_cnn_add_sample.argtypes = [vp, vp, vp]
+    L.dg_cnn_add_sample_mode.argtypes = [vp, vp, vp, i32, i32]
     L.dg_cnn_sample_count.argtypes = [vp]
     L.dg_cnn_set_input_noise.argtypes = [vp, ctypes.c_float]
     L.dg_cnn_fit_normalization.argtypes = [vp]

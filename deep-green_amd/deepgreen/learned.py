@@ -130,8 +130,13 @@ class FluxCNN:
                     "dg_cnn_adam_step")
 
     # training from the solver -----------------------------------------------
-    def add_sample(self, fine, coarse):
-        self._check(self._L.dg_cnn_add_sample(self._h, fine.handle, coarse.handle), "dg_cnn_add_sample")
+    def add_sample(self, fine, coarse, keep_coarse=False, store=True):
+        """One training sample (dg_cnn_add_sample_mode): U^n is the restriction
+        of `fine` or, with keep_coarse, the coarse grid's own state; the fine
+        grid advances two steps and the coarse grid one (plain) step with a
+        common dt; store=False only advances them."""
+        self._check(self._L.dg_cnn_add_sample_mode(self._h, fine.handle, coarse.handle, int(bool(keep_coarse)),
+                                                   int(bool(store))), "dg_cnn_add_sample_mode")
 
     @property
     def samples(self):
@@ -245,9 +250,15 @@ def run_to(solver, T, model=None, guard=4):
             model.step(solver)
 
 
-def make_training_set(model, seeds, n_fine=256, t_end=0.03, every=8, log=None):
+def make_training_set(model, seeds, n_fine=256, t_end=0.03, every=8, log=None, trajectory=False):
     """Samples from plain runs of learned_ic(seed) on n_fine^3 against the 2x
-    coarser grid: one sample every `every` fine steps up to t_end."""
+    coarser grid up to t_end.  trajectory=False: every `every` fine steps the
+    coarse grid is reset to the restricted fine state and one sample taken
+    (the one-step error of the coarse step).  trajectory=True: the coarse grid
+    starts from the restricted initial state and runs its own plain
+    trajectory in step with the fine one (one coarse step per two fine steps,
+    common dt); every `every // 2` coarse steps a sample of its state is kept,
+    with the target that returns it onto the restricted fine trajectory."""
     from . import Solver
     from .setups import learned_ic
     for seed in seeds:
@@ -258,10 +269,16 @@ def make_training_set(model, seeds, n_fine=256, t_end=0.03, every=8, log=None):
         del w
         if log:
             log(f"[learned] seed {seed}: {n_fine}^3 initial condition ready")
-        t = 0.0
+        t, k = 0.0, 0
+        if trajectory:
+            coarse.restrict_from(fine)
         while t < t_end:
-            model.add_sample(fine, coarse)
-            fine.run(max(every - 2, 0))
+            if trajectory:
+                model.add_sample(fine, coarse, keep_coarse=True, store=k % max(every // 2, 1) == 0)
+                k += 1
+            else:
+                model.add_sample(fine, coarse)
+                fine.run(max(every - 2, 0))
             t = clock(fine)[0]
             if log and model.samples % 4 == 0:
                 log(f"[learned] seed {seed}: {model.samples} samples, t = {t:.4f}")

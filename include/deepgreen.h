@@ -50,7 +50,8 @@ typedef enum dg_precision {
 
 typedef enum dg_math {
     DG_MATH_EXACT = 0, /* the reference's operation order, IEEE div/sqrt, no FMA */
-    DG_MATH_FAST = 1   /* same scheme, reassociated with FMA and reciprocals */
+    DG_MATH_FAST = 1   /* same scheme, reassociated with FMA, hardware reciprocals and
+                          reciprocal square roots (FP64: refined by Newton steps) */
 } dg_math;
 
 /* Slope used by the MUSCL-Hancock reconstruction.  NONE is the reference's
@@ -89,6 +90,8 @@ void dg_config_default(dg_config *cfg);
  * allocates the device state (two conserved buffers) and a HIP stream. */
 int dg_create(const dg_config *cfg, dg_ctx **out);
 void dg_destroy(dg_ctx *ctx);
+/* Message of the context's last failure; with ctx NULL, the reason of the
+ * calling thread's last failed dg_create ("null context" if there was none). */
 const char *dg_last_error(const dg_ctx *ctx);
 
 /* Upload a state from host arrays f[0..4] in the reference layout.
@@ -292,6 +295,13 @@ int dg_cnn_adam_step(dg_cnn *m, const float *grads, float lr, float beta1, float
  * flux divergence R = (restricted fine - coarse) / (dt ds^2).  Both grids
  * FP32 and periodic; the fine state advances by two steps. */
 int dg_cnn_add_sample(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse);
+/* dg_cnn_add_sample with two options.  keep_coarse = 1: U^n is the coarse
+ * grid's own current state (its trajectory) instead of the restriction of
+ * `fine`, so R also holds the coarse trajectory's accumulated departure from
+ * the restricted fine one; the coarse grid then advances by its plain step.
+ * store = 0: both grids advance (the fine by two steps, the coarse by one, the
+ * common dt) and no sample is kept. */
+int dg_cnn_add_sample_mode(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse, int keep_coarse, int store);
 int dg_cnn_sample_count(dg_cnn *m);
 /* Standard deviation of Gaussian noise added to the normalized network inputs
  * during dg_cnn_train (0 = none; training-time augmentation). */

@@ -297,3 +297,27 @@ def test_learned_run_after_workspace_growth(emu_lib):
     m2.run(b, 32)
     np.testing.assert_array_equal(got, interior(b.get_state()))
     b.close()
+
+
+def test_trajectory_samples_follow_the_coarse_grids_own_steps(emu_lib):
+    """add_sample(keep_coarse=True): the coarse grid is not reset to the
+    restricted fine state; it takes its own plain step with the common dt
+    (the smaller of its CFL dt and twice the fine one), and store=False keeps
+    no sample."""
+    from deepgreen.learned import FluxCNN
+    m = FluxCNN(seed=10, lib=emu_lib)
+    fine = periodic_solver(emu_lib, 12, 6)
+    coarse = periodic_solver(emu_lib, 6, 7, ds=2.0 / 12)
+    twin = periodic_solver(emu_lib, 6, 7, ds=2.0 / 12)
+    coarse.restrict_from(fine)
+    fine.run(3)                                   # the fine grid moves on; the coarse one keeps its state
+    for store in (False, True, True):
+        U0 = coarse.get_state()
+        dt = min(coarse.next_dt(), 2.0 * fine.next_dt())
+        twin.set_conserved(U0)
+        twin.step_dt(dt)
+        n0 = m.samples
+        m.add_sample(fine, coarse, keep_coarse=True, store=store)
+        assert m.samples == n0 + int(store)
+        np.testing.assert_array_equal(interior(coarse.get_state()), interior(twin.get_state()))
+    fine.close(), coarse.close(), twin.close()
