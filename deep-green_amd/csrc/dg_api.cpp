@@ -331,6 +331,14 @@ int launch_one_step(dg_ctx *c)
     return DG_OK;
 }
 
+// Lift any stop condition of a previous dg_run (a step launched while the
+// scalars say "inactive" carries the state through unchanged).
+int arm_unbounded(dg_ctx *c)
+{
+    hipLaunchKernelGGL(arm_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, (double)INFINITY, (long long)LLONG_MAX, 0);
+    return hipGetLastError() == hipSuccess ? DG_OK : fail(c, DG_ERR_DEVICE, "arm launch");
+}
+
 // mode 0: dt and sums of a freshly set state; 2: sums only
 int reduce_current(dg_ctx *c, int mode)
 {

@@ -49,6 +49,8 @@ int tile_args(StepArgs &a, int prec, int tile_y, int chunk_x, int *chunk_out);
 int create_ctx(const dg_config *cfg, hipStream_t shared_stream, dg_ctx **out, std::string *why = nullptr);
 int set_tiling(dg_ctx *c, int tile_y, int chunk_x);
 int reserve_partials(dg_ctx *c);
+// Clears the stop condition of a previous dg_run so that the next steps advance.
+int arm_unbounded(dg_ctx *c);
 // One step (step kernel, then dt and clock: finalize mode 1); flips c->cur.
 int launch_one_step(dg_ctx *c);
 // Reduction of the current state: mode 0 dt and sums of a fresh state, 2 sums only.

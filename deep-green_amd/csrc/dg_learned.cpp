@@ -653,6 +653,10 @@ int dg_cnn_add_sample_mode(dg_cnn *m, dg_ctx *fine, dg_ctx *coarse, int keep_coa
         for (int d = 0; d < 3; ++d)
             if (m->sn[d] != coarse->args.n[d]) return mfail(m, DG_ERR_INVALID, "samples must share the coarse grid");
     MHIP(m, hipSetDevice(m->device));
+    // both grids must advance: a finished dg_run (a clock query is a run of
+    // zero steps) leaves their scalars inactive, and an inactive step carries
+    // the state through unchanged
+    if (dg::arm_unbounded(fine) || dg::arm_unbounded(coarse)) return mfail(m, DG_ERR_DEVICE, "arm");
     MHIP(m, hipStreamSynchronize(fine->stream));
     MHIP(m, hipStreamSynchronize(coarse->stream));
     const size_t n3 = coarse->n3;
@@ -955,8 +959,10 @@ int dg_step_dt(dg_ctx *c, double dt)
     if (!c || !(dt > 0)) return DG_ERR_INVALID;
     if (!c->have_state) return DG_ERR_STATE;
     if (hipSetDevice(c->cfg.device) != hipSuccess) return DG_ERR_DEVICE;
+    int rc = dg::arm_unbounded(c);   // as dg_step: a finished dg_run must not turn this step into a no-op
+    if (rc) return rc;
     if (dg::launch_set_dt(c->sc, dt, c->stream)) return DG_ERR_DEVICE;
-    int rc = dg::launch_one_step(c);
+    rc = dg::launch_one_step(c);
     if (rc) return rc;
     return hipStreamSynchronize(c->stream) == hipSuccess ? DG_OK : DG_ERR_DEVICE;
 }

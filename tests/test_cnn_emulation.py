@@ -321,3 +321,26 @@ def test_trajectory_samples_follow_the_coarse_grids_own_steps(emu_lib):
         assert m.samples == n0 + int(store)
         np.testing.assert_array_equal(interior(coarse.get_state()), interior(twin.get_state()))
     fine.close(), coarse.close(), twin.close()
+
+
+def test_sample_and_step_dt_advance_after_a_finished_run(emu_lib):
+    """A finished dg_run (a clock query is a run of zero steps) leaves a
+    context's scalars inactive; training samples and dg_step_dt must still
+    advance their grids (an inactive step carries the state unchanged)."""
+    from deepgreen.learned import FluxCNN
+    m = FluxCNN(seed=11, lib=emu_lib)
+    fine = periodic_solver(emu_lib, 12, 8)
+    coarse = periodic_solver(emu_lib, 6, 9, ds=2.0 / 12)
+    fine.run(2)
+    _, n0 = fine.run(0)
+    U0 = interior(fine.get_state())
+    m.add_sample(fine, coarse)
+    _, n1 = fine.run(0)
+    assert n1 == n0 + 2
+    assert not np.array_equal(interior(fine.get_state()), U0)
+    coarse.run(0)
+    W0 = interior(coarse.get_state())
+    coarse.step_dt(1e-4)
+    assert coarse.run(0)[1] >= 1
+    assert not np.array_equal(interior(coarse.get_state()), W0)
+    fine.close(), coarse.close()
