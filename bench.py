@@ -347,9 +347,9 @@ def main():
     ap.add_argument("--workload", default="khi", choices=["khi", "ensemble", "learned", "blast"])
     ap.add_argument("--n-blast", type=int, default=256)
     ap.add_argument("--n-learned", type=int, default=128)
-    ap.add_argument("--train-ics", type=int, default=4)
+    ap.add_argument("--train-ics", type=int, default=32)
     ap.add_argument("--train-steps", type=int, default=2000)
-    ap.add_argument("--sample-every", type=int, default=8)
+    ap.add_argument("--sample-every", type=int, default=4)
     ap.add_argument("--train-mode", default="reset", choices=["reset", "trajectory"],
                     help="learned workload: samples of restricted fine states (reset) or of the coarse grid's own "
                          "plain trajectory (trajectory)")
