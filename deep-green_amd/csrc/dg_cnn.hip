@@ -43,10 +43,35 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------- helpers --
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU(z) = z Phi(z) = 0.5 z (1 + erf(z / sqrt 2)), with erf from Abramowitz &
+// Stegun 7.1.26 (rational approximation in t = 1 / (1 + p|x|) times exp(-x^2),
+// absolute error <= 1.5e-7, far below the bf16 rounding of the activations):
+// one reciprocal and one exp2 instead of a library erff.
+struct ErfExp {
+    float erf, e;   // erf(x), exp(-x^2)
+};
+__device__ __forceinline__ ErfExp erf_as(float x)
+{
+    const float ax = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, ax, 1.0f));
+    const float e = exp2f(-1.44269504088896341f * x * x);
+    float p = fmaf(1.061405429f, t, -1.453152027f);
+    p = fmaf(p, t, 1.421413741f);
+    p = fmaf(p, t, -0.284496736f);
+    p = fmaf(p, t, 0.254829592f);
+    p *= t;
+    return {copysignf(fmaf(-p, e, 1.0f), x), e};
+}
+__device__ __forceinline__ float gelu(float z)
+{
+    const float h = 0.5f * z;
+    return fmaf(h, erf_as(z * 0.70710678118654752f).erf, h);
+}
+// GELU'(z) = Phi(z) + z phi(z), phi(z) = exp(-z^2 / 2) / sqrt(2 pi)
 __device__ __forceinline__ float gelu_grad(float z)
 {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+    const ErfExp r = erf_as(z * 0.70710678118654752f);
+    return fmaf(0.5f, r.erf, 0.5f) + z * 0.39894228040143268f * r.e;
 }
 
 // 32-bit integer hash (xorshift-multiply finalizer)
@@ -213,6 +238,268 @@ conv_kernel(const ConvGeom g, const bf16 *__restrict__ X, const bf16x8 *__restri
                 }
             }
         }
+    }
+}
+
+// ------------------------------------------------ conv (MFMA, LDS ring) --
+// The same convolutions with the activations staged in LDS.  A workgroup
+// (4 waves, one per SIMD) owns a rectangle of RY x RZ output (y, z)
+// positions, flattened row-major (at most 256: 64 per wave, NT MFMA tiles of
+// TILE positions), and marches it along x over a chunk of output planes.
+// Output plane x reads the input planes x - PAD + {0, 1, 2}; they sit in a
+// ring of 4 LDS slots, and the plane needed next is copied into the fourth
+// slot while the current plane is computed (one barrier per plane; its loads
+// are issued before the plane's MFMAs and written to LDS after them).  A slot
+// holds (RY + 2) rows of RING_ROWP cells; a cell
+// is CIN bf16 channels in 16-B chunks, padded by one chunk for CIN >= 16 so
+// that the 128-bit reads of consecutive cells hit distinct banks.  Cells
+// outside the input box load zeros (the full convolution's padding).  The
+// weights stay in VGPRs for the whole kernel, so LDS carries only the
+// activations: one 1-KB read per MFMA.
+constexpr int RING_WAVES = 4;
+constexpr int RING_POS = 64 * RING_WAVES;   // output positions per workgroup plane
+constexpr int RING_ROWP = 48;               // cells per LDS row: RZ + 2 <= RING_ROWP
+constexpr int RING_RMAX = 9;                // LDS rows per slot: RY + 2 <= RING_RMAX
+constexpr int RING_SLOTS = 4;
+constexpr int RING_DEPTH = 3;               // k-steps of B operands loaded ahead
+
+struct RingGeom {
+    ConvGeom g;
+    int ry, rz;        // output rectangle of a workgroup: ry * rz <= RING_POS
+    int ny, nz;        // rectangles along y, z
+    int cx, nxc;       // output planes per work item, chunks along x
+    long long items;   // B * nxc * ny * nz
+};
+
+template <int CIN> struct RingCell {
+    static constexpr int NC = CIN / 8;                   // 16-B chunks of channels per cell
+    static constexpr int CS = NC == 1 ? 1 : NC + 1;      // chunk stride of a cell
+    static constexpr int CHUNKS = RING_RMAX * RING_ROWP * CS;
+    static constexpr int BLOCKS = (CHUNKS + 63) / 64;    // 1-KB blocks (one load instruction per wave)
+    static constexpr int SLOT = BLOCKS * 1024;           // bytes per slot
+};
+
+// zeros for the cells outside the input box (never written)
+static __device__ __attribute__((aligned(16))) uint32_t ring_zero[4];
+
+// Filling a slot with input plane xi of batch item b (rows y0 .. y0 + rows - 1,
+// cells z0 .. z0 + cells - 1): each thread loads its share of 16-B chunks into
+// registers (fill_load, at the top of a plane) and writes them to LDS
+// (fill_store, after the plane's MFMAs), so the loads' latency hides under
+// the MFMAs and the LDS reads keep exact wait counts.
+template <int CIN> struct RingFill {
+    static constexpr int NC = CIN / 8;
+    static constexpr int PER = (RING_RMAX * RING_ROWP * NC + 64 * RING_WAVES - 1) / (64 * RING_WAVES);
+};
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <int CIN>
+__device__ __forceinline__ void fill_load(u32x4 (&v)[RingFill<CIN>::PER], const bf16 *__restrict__ X,
+                                          const ConvGeom &g, int b, int xi, int y0, int z0, int rows, int cells,
+                                          int tid)
+{
+    typedef RingFill<CIN> F;
+    const bool xin = xi >= 0 && xi < g.in[0];
+    const long long pbase = ((long long)b * g.in[0] + xi) * g.in[1];
+#pragma unroll
+    for (int k = 0; k < F::PER; ++k) {
+        const int idx = k * 64 * RING_WAVES + tid;
+        const int cell = idx / F::NC, q = idx % F::NC;
+        const int r = cell / RING_ROWP, c = cell % RING_ROWP;
+        const int yi = y0 + r, zi = z0 + c;
+        const bool in = xin && r < rows && c < cells && yi >= 0 && yi < g.in[1] && zi >= 0 && zi < g.in[2];
+        const u32x4 *src = in ? reinterpret_cast<const u32x4 *>(X + ((pbase + yi) * g.in[2] + zi) * CIN + q * 8)
+                              : reinterpret_cast<const u32x4 *>(ring_zero);
+        v[k] = *src;
+    }
+}
+
+template <int CIN>
+__device__ __forceinline__ void fill_store(char *slot, const u32x4 (&v)[RingFill<CIN>::PER], int rows, int cells,
+                                           int tid)
+{
+    typedef RingFill<CIN> F;
+#pragma unroll
+    for (int k = 0; k < F::PER; ++k) {
+        const int idx = k * 64 * RING_WAVES + tid;
+        const int cell = idx / F::NC, q = idx % F::NC;
+        const int r = cell / RING_ROWP, c = cell % RING_ROWP;
+        if (r < rows && c < cells)
+            *reinterpret_cast<u32x4 *>(slot + (cell * RingCell<CIN>::CS + q) * 16) = v[k];
+    }
+}
+
+template <int CIN, int COUT, int EPI, int PAD>
+__global__ void __launch_bounds__(64 * RING_WAVES) __attribute__((amdgpu_waves_per_eu(1, 1)))
+conv_ring_kernel(const RingGeom rg, const bf16 *__restrict__ X, const bf16x8 *__restrict__ Wp,
+                 const float *__restrict__ bias, float *__restrict__ Zout, bf16 *__restrict__ Xout,
+                 float *__restrict__ Yout, const float *__restrict__ Zprev)
+{
+    typedef ConvCfg<CIN, COUT> C;
+    typedef Mfma<COUT> M;
+    typedef RingCell<CIN> RC;
+    constexpr int NT = C::NT;
+    // a k-step lies inside one tap unless CIN < KSTEP (then each lane group has its own tap)
+    constexpr bool ALIGNED = CIN % C::KSTEP == 0;
+    static_assert(ALIGNED || (CIN == 8 && C::KSTEP == 16), "k-step layout");
+    __shared__ __attribute__((aligned(1024))) char ring[RING_SLOTS * RC::SLOT];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane % C::TILE, part = lane / C::TILE;
+    const ConvGeom &g = rg.g;
+
+    bf16x8 wa[C::KS];
+#pragma unroll
+    for (int s = 0; s < C::KS; ++s) wa[s] = Wp[s * 64 + lane];
+    // this lane's output channels' biases (no memory wait in the epilogue)
+    float bv[M::NACC];
+#pragma unroll
+    for (int gq = 0; gq < M::NACC / 4; ++gq)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bv[4 * gq + q] = EPI == EPI_BWD ? 0.0f : bias[M::row(4 * gq, lane) + q];
+
+    const int RY = rg.ry, RZ = rg.rz, npos = RY * RZ;
+    const int o0 = g.out[0], o1 = g.out[1], o2 = g.out[2];
+    for (long long item = blockIdx.x; item < rg.items; item += gridDim.x) {
+        long long r = item;
+        const int iz = (int)(r % rg.nz); r /= rg.nz;
+        const int iy = (int)(r % rg.ny); r /= rg.ny;
+        const int xc = (int)(r % rg.nxc);
+        const int b = (int)(r / rg.nxc);
+        const int y0 = iy * RY, z0 = iz * RZ;
+        const int xs = xc * rg.cx, xe = min(xs + rg.cx, o0);
+
+        // this lane's positions: LDS byte offset within a slot, output (y, z) index
+        int lb[NT], oyz[NT];
+        bool ok[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int p = (wave * NT + t) * C::TILE + col;
+            const int pp = p < npos ? p : 0;
+            const int py = pp / RZ, pz = pp - py * RZ;
+            ok[t] = p < npos && y0 + py < o1 && z0 + pz < o2;
+            lb[t] = (py * RING_ROWP + pz) * RC::CS * 16 + (ALIGNED ? part * 16 : 0);
+            oyz[t] = (y0 + py) * o2 + z0 + pz;
+        }
+
+        // prologue: the first three input planes
+        u32x4 fv[RingFill<CIN>::PER];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            fill_load<CIN>(fv, X, g, b, xs + j - PAD, y0 - PAD, z0 - PAD, RY + 2, RZ + 2, threadIdx.x);
+            fill_store<CIN>(ring + ((xs + j) & 3) * RC::SLOT, fv, RY + 2, RZ + 2, threadIdx.x);
+        }
+        __syncthreads();
+
+        // The epilogue of plane x runs in slices inside the k-loop of plane x + 1
+        // (its VALU work overlaps the MFMAs; one wave per SIMD has no other
+        // wave to hide it behind); the last plane's runs after the loop.
+        constexpr int NEPI = NT * (M::NACC / 4);
+        auto epilogue = [&](const typename M::acc_t *A, long long mb, int e) {
+            const int t = e / (M::NACC / 4), gq = e % (M::NACC / 4);
+            if (!ok[t]) return;
+            const long long m = mb + oyz[t];
+            const int ch = M::row(4 * gq, lane);
+            float v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = A[t][4 * gq + q];
+            if (EPI == EPI_BWD) {
+                const f32x4 zp = *reinterpret_cast<const f32x4 *>(Zprev + m * COUT + ch);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[q] *= gelu_grad(zp[q]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[q] += bv[4 * gq + q];
+            }
+            if (EPI == EPI_TRAIN) {
+                f32x4 z;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) z[q] = v[q];
+                *reinterpret_cast<f32x4 *>(Zout + m * COUT + ch) = z;
+            }
+            if (EPI == EPI_OUT) {
+                f32x4 y;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) y[q] = v[q];
+                *reinterpret_cast<f32x4 *>(Yout + m * COUT + ch) = y;
+            } else {
+                bf16x4 xo;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) xo[q] = (bf16)(EPI == EPI_BWD ? v[q] : gelu(v[q]));
+                *reinterpret_cast<bf16x4 *>(Xout + m * COUT + ch) = xo;
+            }
+        };
+        const long long plane_cells = (long long)o1 * o2;
+        typename M::acc_t prv[NT];
+        long long pmb = -1;   // output index of (plane of prv, y = z = 0); -1: none pending
+
+        for (int x = xs; x < xe; ++x) {
+            const bool more = x + 1 < xe;
+            if (more) fill_load<CIN>(fv, X, g, b, x + 3 - PAD, y0 - PAD, z0 - PAD, RY + 2, RZ + 2, threadIdx.x);
+            const char *sb[3];
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) sb[dx] = ring + ((x + dx) & 3) * RC::SLOT;
+
+            typename M::acc_t acc[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int i = 0; i < M::NACC; ++i) acc[t][i] = 0.0f;
+
+            // B operand of k-step s for tile t (8 channels of one tap of this lane's position)
+            auto load_b = [&](int s, int t) -> bf16x8 {
+                if (ALIGNED) {
+                    const int k = s * C::KSTEP, tap = k / CIN, q = (k % CIN) / 8;
+                    const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;
+                    const int off = ((dy * RING_ROWP + dz) * RC::CS + q) * 16;
+                    return *reinterpret_cast<const bf16x8 *>(sb[dx] + lb[t] + off);
+                } else {
+                    // CIN 8, KSTEP 16: lanes of part 0 read tap 2s, part 1 tap 2s + 1 (zero past tap 26)
+                    const int ta = 2 * s, tb = 2 * s + 1;
+                    const int offa = ((ta / 3) % 3 * RING_ROWP + ta % 3) * 16;
+                    const int offb = ((tb / 3) % 3 * RING_ROWP + tb % 3) * 16;
+                    const char *pa = sb[ta / 9] + lb[t] + offa;
+                    if (tb >= 27) return part == 0 ? *reinterpret_cast<const bf16x8 *>(pa) : zero8();
+                    const char *pb = sb[tb / 9] + lb[t] + offb;
+                    return *reinterpret_cast<const bf16x8 *>(part == 0 ? pa : pb);
+                }
+            };
+            // B operands of RING_DEPTH k-steps in flight (LDS latency under the MFMAs);
+            // the scheduling barriers keep the issue order as written
+            bf16x8 bq[RING_DEPTH][NT];
+#pragma unroll
+            for (int d = 0; d < RING_DEPTH; ++d)
+                if (d < C::KS)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) bq[d][t] = load_b(d, t);
+            // NEPI segments of SEG k-steps, each followed by one epilogue slice
+            constexpr int SEG = (C::KS + NEPI - 1) / NEPI;
+#pragma unroll
+            for (int e = 0; e < NEPI; ++e) {
+#pragma unroll
+                for (int j = 0; j < SEG; ++j) {
+                    const int s = e * SEG + j;
+                    if (s < C::KS) {
+#pragma unroll
+                        for (int t = 0; t < NT; ++t) acc[t] = M::op(wa[s], bq[s % RING_DEPTH][t], acc[t]);
+                        if (s + RING_DEPTH < C::KS)
+#pragma unroll
+                            for (int t = 0; t < NT; ++t) bq[s % RING_DEPTH][t] = load_b(s + RING_DEPTH, t);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                }
+                if (pmb >= 0) epilogue(prv, pmb, e);
+            }
+#pragma unroll
+            for (int t = 0; t < NT; ++t) prv[t] = acc[t];
+            pmb = ((long long)b * o0 + x) * plane_cells;
+            // the slot of plane x - 1 is free since the last barrier
+            if (more) fill_store<CIN>(ring + ((x + 3) & 3) * RC::SLOT, fv, RY + 2, RZ + 2, threadIdx.x);
+            __syncthreads();
+        }
+        if (pmb >= 0)
+#pragma unroll
+            for (int e = 0; e < NEPI; ++e) epilogue(prv, pmb, e);
     }
 }
 
@@ -560,18 +847,61 @@ static int grid_for(long long n, long long per, int cap)
     return (int)b;
 }
 
+static int cu_count()
+{
+    static int n = 0;
+    if (!n) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            n = v;
+        else
+            n = 256;
+    }
+    return n;
+}
+
+// Rectangles of at most RING_POS positions with RZ + 2 <= RING_ROWP and
+// RY + 2 <= RING_RMAX, balanced over the output extents; x chunks so that
+// there are about four work items per CU.
+static RingGeom ring_geom(const ConvGeom &g)
+{
+    RingGeom r;
+    r.g = g;
+    const int o0 = g.out[0], o1 = g.out[1], o2 = g.out[2];
+    r.nz = (o2 + RING_ROWP - 3) / (RING_ROWP - 2);
+    r.rz = (o2 + r.nz - 1) / r.nz;
+    int ry = RING_POS / r.rz;
+    if (ry > RING_RMAX - 2) ry = RING_RMAX - 2;
+    if (ry > o1) ry = o1;
+    if (ry < 1) ry = 1;
+    r.ny = (o1 + ry - 1) / ry;
+    r.ry = (o1 + r.ny - 1) / r.ny;
+    const long long cols = (long long)g.B * r.ny * r.nz;
+    long long nxc = (4LL * cu_count() + cols - 1) / cols;
+    if (nxc > o0) nxc = o0;
+    if (nxc < 1) nxc = 1;
+    r.cx = (int)((o0 + nxc - 1) / nxc);
+    r.nxc = (o0 + r.cx - 1) / r.cx;
+    r.items = cols * r.nxc;
+    return r;
+}
+
 int conv(int layer, int mode, int B, const int in[3], const void *X, const void *Wp, const float *bias, float *Z,
          void *Xout, float *Y, const float *Zprev, hipStream_t s)
 {
     const int pad = mode == CONV_BWD ? 2 : 0;
     const ConvGeom g = geom(B, in, pad);
-    const int blocks = grid_for(g.ncells, 64 * CONV_WAVES, 8192);
+    for (int d = 0; d < 3; ++d)
+        if (g.out[d] < 1) return -1;
+    const RingGeom rg = ring_geom(g);
+    const int blocks = (int)(rg.items < cu_count() ? rg.items : cu_count());
     const bf16 *x = static_cast<const bf16 *>(X);
     const bf16x8 *w = static_cast<const bf16x8 *>(Wp);
     bf16 *xo = static_cast<bf16 *>(Xout);
 #define DG_CONV(CI, CO, E, P)                                                                                 \
-    hipLaunchKernelGGL((conv_kernel<CI, CO, E, P>), dim3(blocks), dim3(64 * CONV_WAVES), 0, s, g, x, w, bias, Z, xo, \
-                       Y, Zprev)
+    hipLaunchKernelGGL((conv_ring_kernel<CI, CO, E, P>), dim3(blocks), dim3(64 * RING_WAVES), 0, s, rg, x, w, bias, Z, \
+                       xo, Y, Zprev)
     if (layer == 1) {
         if (mode == CONV_TRAIN) DG_CONV(8, 32, EPI_TRAIN, 0);
         else if (mode == CONV_INFER) DG_CONV(8, 32, EPI_INFER, 0);

@@ -316,7 +316,7 @@ int prepare_infer(dg_cnn *m, dg_ctx *c)
         if (m->flags) (void)hipFree(m->flags);
         m->flags = nullptr;
         m->flags_n = 0;
-        MHIP(m, hipMalloc((void **)&m->flags, c->n3));
+        MHIP(m, hipMalloc((void **)&m->flags, dg::corr_flags_bytes(c->n3)));
         m->flags_n = c->n3;
     }
     // workspaces of infer(), so that a captured step allocates nothing

@@ -1131,7 +1131,10 @@ cons_to_prim_kernel(const StepArgs A, const Fields<const T> U, const Fields<T> W
 // the face flux correction telescopes, so the totals are unchanged.  A cell
 // whose corrected state would have rho <= 0 or p <= 0 is flagged; every face
 // of a flagged cell keeps the plain flux (m = 0), so each face still carries
-// one flux for both of its cells.
+// one flux for both of its cells.  Keeping those faces plain changes the
+// updates of the flagged cells' neighbours, so two more passes (run only when
+// the first flagged a cell) flag every cell whose state is invalid under the
+// current masks.
 struct CorrScale { float s[5]; };
 
 __device__ __forceinline__ long long wrap_index(int x, int y, int z, const StepArgs &A)
@@ -1166,21 +1169,35 @@ __device__ __forceinline__ void corr_delta(const StepArgs &A, const float *__res
     }
 }
 
-template <bool FAST>
+__global__ void zero_count_kernel(int *cnt) { *cnt = 0; }
+
+// MASKED = false: flags[i] = corrected state of i invalid with every face
+// corrected, and cnt counts the flagged cells.  MASKED = true (skipped when
+// cnt is 0): flags[i] = prev[i] or state of i invalid with the faces of the
+// cells flagged in prev kept plain.
+template <bool FAST, bool MASKED>
 __global__ void __launch_bounds__(256)
 corr_flag_kernel(const StepArgs A, const Fields<const float> U, const float *__restrict__ G, const CorrScale S,
-                 const Scalars *__restrict__ sc, unsigned char *__restrict__ flags)
+                 const Scalars *__restrict__ sc, const unsigned char *__restrict__ prev,
+                 unsigned char *__restrict__ flags, int *__restrict__ cnt)
 {
+    if (MASKED && *cnt == 0) return;
     const float ds = float(A.ds), V = ds * ds * ds, gm1 = float(A.gamma) - 1.0f;
     const float coef = sc->active ? -float(sc->dt) * ds * ds : 0.0f;   // an inactive step changes nothing
     const long long n3 = (long long)A.n[0] * A.n[1] * A.n[2];
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long long)gridDim.x * blockDim.x) {
         const int z = (int)(i % A.n[2]), y = (int)((i / A.n[2]) % A.n[1]), x = (int)(i / ((long long)A.n[1] * A.n[2]));
         float d[5], u[5], w[5];
-        corr_delta(A, G, S, coef, nullptr, x, y, z, d);
+        corr_delta(A, G, S, coef, MASKED ? prev : nullptr, x, y, z, d);
         for (int q = 0; q < 5; ++q) u[q] = U.f[q][i] + d[q];
         to_prim<float, FAST>(u, V, 1.0f / V, gm1, w);
-        flags[i] = (w[0] > 0.0f && w[4] > 0.0f) ? 0 : 1;
+        const bool bad = !(w[0] > 0.0f && w[4] > 0.0f);
+        if (MASKED) {
+            flags[i] = (prev[i] || bad) ? 1 : 0;
+        } else {
+            flags[i] = bad ? 1 : 0;
+            if (bad) atomicAdd(cnt, 1);
+        }
     }
 }
 
@@ -1495,6 +1512,8 @@ static int cells_grid(long long n3)
     return (int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
 }
 
+size_t corr_flags_bytes(long long n3) { return (size_t)((2 * n3 + 63) / 64) * 64 + 64; }
+
 int launch_corr(int fast, const StepArgs &a, void *const u[5], const float *G, const float scale[5], const Scalars *sc,
                 unsigned char *flags, double *partials, hipStream_t s)
 {
@@ -1502,12 +1521,21 @@ int launch_corr(int fast, const StepArgs &a, void *const u[5], const float *G, c
     for (int q = 0; q < 5; ++q) S.s[q] = scale[q];
     const int nb = reduce_blocks(a);
     const long long n3 = (long long)a.n[0] * a.n[1] * a.n[2];
+    // flags: the final flags in [0, n3), the second pass's in [n3, 2 n3), then the count
+    unsigned char *fa = flags, *fb = flags + n3;
+    int *cnt = reinterpret_cast<int *>(flags + corr_flags_bytes(n3) - 64);
+    const dim3 g(cells_grid(n3)), b(256);
+    hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, s, cnt);
     if (fast) {
-        hipLaunchKernelGGL((corr_flag_kernel<true>), dim3(cells_grid(n3)), dim3(256), 0, s, a, cfields<float>(u), G, S, sc, flags);
-        hipLaunchKernelGGL((corr_apply_kernel<true>), dim3(nb), dim3(256), 0, s, a, mfields<float>(u), G, S, sc, flags, partials);
+        hipLaunchKernelGGL((corr_flag_kernel<true, false>), g, b, 0, s, a, cfields<float>(u), G, S, sc, nullptr, fa, cnt);
+        hipLaunchKernelGGL((corr_flag_kernel<true, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fa, fb, cnt);
+        hipLaunchKernelGGL((corr_flag_kernel<true, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fb, fa, cnt);
+        hipLaunchKernelGGL((corr_apply_kernel<true>), dim3(nb), b, 0, s, a, mfields<float>(u), G, S, sc, fa, partials);
     } else {
-        hipLaunchKernelGGL((corr_flag_kernel<false>), dim3(cells_grid(n3)), dim3(256), 0, s, a, cfields<float>(u), G, S, sc, flags);
-        hipLaunchKernelGGL((corr_apply_kernel<false>), dim3(nb), dim3(256), 0, s, a, mfields<float>(u), G, S, sc, flags, partials);
+        hipLaunchKernelGGL((corr_flag_kernel<false, false>), g, b, 0, s, a, cfields<float>(u), G, S, sc, nullptr, fa, cnt);
+        hipLaunchKernelGGL((corr_flag_kernel<false, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fa, fb, cnt);
+        hipLaunchKernelGGL((corr_flag_kernel<false, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fb, fa, cnt);
+        hipLaunchKernelGGL((corr_apply_kernel<false>), dim3(nb), b, 0, s, a, mfields<float>(u), G, S, sc, fa, partials);
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }

@@ -131,6 +131,7 @@ inline hipError_t hipSetDevice(int) { return hipSuccess; }
 // a __device__ array is a host array here: its address is the symbol itself
 inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
+inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
 inline hipError_t hipMalloc(void **p, size_t n) { *p = std::calloc(1, n ? n : 1); return *p ? hipSuccess : 2; }
 template <typename T> inline hipError_t hipMalloc(T **p, size_t n) { return hipMalloc((void **)p, n); }
 inline hipError_t hipFree(void *p) { std::free(p); return hipSuccess; }
@@ -230,6 +231,14 @@ inline emu_f32x4 __builtin_amdgcn_mfma_f32_16x16x32_bf16(emu_bf16x8 a, emu_bf16x
     emu::sync();
     return d;
 }
+// Direct global->LDS load (size bytes per lane): lane l's bytes land at
+// lds + size * l, visible once the issuing wave waits and the block syncs.
+template <typename P> inline void __builtin_amdgcn_global_load_lds(const void *src, P *lds, int size, int, int)
+{
+    std::memcpy((char *)lds + (threadIdx.x & 63) * size, src, size);
+}
+inline void __builtin_amdgcn_s_waitcnt(int) {}
+inline void __builtin_amdgcn_sched_barrier(int) {}
 inline int atomicAdd(int *p, int v)
 {
     const int old = *p;   // fibers of one host thread: no concurrent access

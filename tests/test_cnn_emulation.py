@@ -153,13 +153,21 @@ def correction_reference(Up, G, dt, ds, scale, gamma):
             d += np.moveaxis(mh * Gd - ml * Glo, -1, 0)
         return -dt * ds * ds * np.asarray(scale, np.float64)[:, None, None, None] * d
 
-    cand = Up + delta()
-    rho = cand[0] / V
-    p = (cand[4] / V - 0.5 * (cand[1] ** 2 + cand[2] ** 2 + cand[3] ** 2) / (rho * V * V)) * gm1
-    flag = ~((rho > 0) & (p > 0))
-    mh = [~(flag | np.roll(flag, -1, axis=a)) for a in range(3)]
-    ml = [~(flag | np.roll(flag, 1, axis=a)) for a in range(3)]
-    return Up + delta(mh, ml), flag
+    def invalid(U):
+        rho = U[0] / V
+        p = (U[4] / V - 0.5 * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / (rho * V * V)) * gm1
+        return ~((rho > 0) & (p > 0))
+
+    def masks(flag):
+        return ([~(flag | np.roll(flag, -1, axis=a)) for a in range(3)],
+                [~(flag | np.roll(flag, 1, axis=a)) for a in range(3)])
+
+    flag = invalid(Up + delta())
+    if flag.any():
+        # two passes: cells invalid once the flagged cells' faces keep the plain flux
+        for _ in range(2):
+            flag = flag | invalid(Up + delta(*masks(flag)))
+    return Up + delta(*masks(flag)), flag
 
 
 def test_learned_correction_fallback_and_conservation(emu_lib):
@@ -189,6 +197,37 @@ def test_learned_correction_fallback_and_conservation(emu_lib):
     np.testing.assert_array_equal(got[:, 2, 3, 1], Up[:, 2, 3, 1].astype(np.float32))
     sums1 = a.conservation_sums()
     np.testing.assert_allclose(sums1, sums0, rtol=0, atol=1e-5 * np.abs(Up).sum(axis=(1, 2, 3)).max())
+    a.close(), p.close()
+
+
+def test_learned_correction_fallback_reaches_the_neighbours(emu_lib):
+    """Two huge mass fluxes in a row along x: cell (2, 3, 1) empties through
+    its +x face and is flagged; cell (3, 3, 1) passes the mass on through its
+    own +x face (net zero with every face corrected), but once the face it
+    shares with the flagged cell keeps the plain flux it would empty too, so
+    the masked pass flags it as well.  The step is then the plain step on
+    every cell those faces touch, and the state stays valid."""
+    from deepgreen.learned import FluxCNN
+    m = FluxCNN(seed=6, lib=emu_lib)
+    n, ds = 6, 1.0 / 6
+    a, p = periodic_solver(emu_lib, n, 2), periodic_solver(emu_lib, n, 2)
+    sums0 = a.conservation_sums()
+    G = np.zeros((n, n, n, 16), np.float32)
+    G[2, 3, 1, 0] = 1e4
+    G[3, 3, 1, 0] = 1e4
+    dt = m.step_given(a, G)
+    assert dt == p.compute_step()
+    Up = interior(p.get_state()).astype(np.float64)
+    scale = m.get_normalization()[10:15]
+    expect, flag = correction_reference(Up, G, dt, ds, scale, 1.4)
+    assert flag.sum() == 2 and flag[2, 3, 1] and flag[3, 3, 1]
+    assert m.flagged() == 2
+    got = interior(a.get_state())
+    # every nonzero face belongs to a flagged cell: the learned step is the plain step
+    np.testing.assert_array_equal(got, Up.astype(np.float32))
+    np.testing.assert_allclose(got, expect, rtol=0, atol=0)
+    assert (got[0] > 0).all() and np.isfinite(a.next_dt())
+    np.testing.assert_allclose(a.conservation_sums(), sums0, rtol=0, atol=1e-5 * np.abs(Up).sum(axis=(1, 2, 3)).max())
     a.close(), p.close()
 
 
