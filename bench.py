@@ -21,6 +21,8 @@ Prints one JSON line (rank 0).  Fields beyond the contract:
                 (FETCH_SIZE + WRITE_SIZE, copy-calibrated) where a committed
                 profile summary holds it.
   cpu_baseline  the reference CPU path on this host, on a bounded sample.
+  secondary     (default KHI run) short runs of the other configurations:
+                KHI 512^3 FP64 fast and FP32 exact, the ensemble, the blast.
 """
 import argparse
 import json
@@ -339,6 +341,46 @@ def bench_blast(args):
     }
 
 
+def secondary(args):
+    """Short runs of the other benchmark configurations after the headline
+    one (the default run only): KHI 512^3 in FP64 fast math and in FP32
+    exact math, the 256-grid ensemble and the 256^3 blast (FP64 against
+    FP32-bulk).  Each entry holds its own throughput; a configuration that
+    fails records the error instead of ending the run."""
+    import copy
+    out = {}
+
+    def run(name, fn, **kw):
+        a = copy.copy(args)
+        for k, v in kw.items():
+            setattr(a, k, v)
+        t0 = time.perf_counter()
+        try:
+            r = fn(a)
+        except Exception as e:  # noqa: BLE001 - recorded, the headline line still prints
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+            return
+        entry = {"value": round(r["value"], 3), "unit": "Gcell-updates/s", "steps": a.steps,
+                 "ms_per_step": round(r["s_per_step"] * 1e3, 4), "wall_s": round(time.perf_counter() - t0, 1)}
+        if r.get("roofline"):
+            entry["kernel_ms"] = r["roofline"]["kernel_ms"]
+            entry["frac_hbm"] = r["roofline"]["frac"]
+            entry["traffic"] = r["roofline"]["traffic"]
+        cfg = r["config"]
+        for k in ("fp64_gcells", "fp32_bulk_gcells", "fp32_bulk_fast_gcells", "fp32_bulk_error_vs_fp64", "members",
+                  "cells"):
+            if k in cfg:
+                entry[k] = cfg[k]
+        out[name] = entry
+        print(f"[bench] secondary {name}: {entry['value']} Gcell-updates/s", file=sys.stderr, flush=True)
+
+    run("khi512_fp64_fast", bench_khi, precision="fp64", math="fast", steps=48, warmup=16)
+    run("khi512_fp32_exact", bench_khi, precision="fp32", math="exact", steps=48, warmup=16)
+    run("ensemble256_fp32_fast", bench_ensemble, workload="ensemble", math="fast", steps=40)
+    run("blast256_fp64_vs_fp32bulk", bench_blast, workload="blast", steps=500)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -362,6 +404,8 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32_bulk", "fp64"])
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="default KHI run: skip the short runs of the other configurations")
     ap.add_argument("--cpu-sample", type=int, default=256, help="edge of the KHI sample the CPU path runs")
     ap.add_argument("--cpu-steps", type=int, default=12, help="steps of the CPU sample (~10 s on 16 cores)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -392,6 +436,10 @@ def main():
         r = bench_blast(args)
     else:
         r = bench_khi(args)
+    sec = None
+    if args.workload == "khi" and not args.no_secondary and args.n == 512 and args.precision == "fp32" \
+            and args.math == "fast":
+        sec = secondary(args)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_ensemble() if args.workload == "ensemble" else cpu_baseline(args.cpu_sample,
@@ -413,6 +461,8 @@ def main():
         "roofline": r["roofline"],
         "cpu_baseline": cpu,
     }
+    if sec is not None:
+        out["secondary"] = sec
     print(json.dumps(out), flush=True)
 
 

@@ -9,8 +9,9 @@
 //   forward / input-gradient  D[cout][cell] = sum_k W[k][cout] * X[cell, k],
 //                             k = tap * CIN + cin, tap = (dx*3 + dy)*3 + dz;
 //                             A = weights (pre-packed in MFMA fragment order,
-//                             staged in LDS), B = activations gathered from
-//                             HBM (channels-last, 16 B per lane per k-step);
+//                             held in VGPRs), B = activations (channels-last)
+//                             read from a ring of input planes in LDS, 16 B
+//                             per lane per k-step (conv_ring_kernel);
 //   weight gradient           D[cout][k] = sum_cells dZ[cell][cout] * X[cell + tap][cin],
 //                             both operands staged channel-major in LDS per
 //                             brick of cells; one partial per workgroup,
@@ -126,120 +127,6 @@ template <int CIN, int COUT> struct ConvCfg {
 // EPI_TRAIN: Z = acc + b (fp32 out), X = bf16(gelu(Z));  EPI_INFER: X only;
 // EPI_OUT: Y = acc + b (fp32, COUT channels);  EPI_BWD: X = bf16(acc * gelu'(Zprev)).
 enum { EPI_TRAIN = 0, EPI_INFER = 1, EPI_OUT = 2, EPI_BWD = 3 };
-
-constexpr int CONV_WAVES = 8;   // waves per workgroup (they share the weights in LDS)
-
-template <int CIN, int COUT, int EPI, int PAD>
-__global__ void __launch_bounds__(64 * CONV_WAVES)
-conv_kernel(const ConvGeom g, const bf16 *__restrict__ X, const bf16x8 *__restrict__ Wp,
-            const float *__restrict__ bias, float *__restrict__ Zout, bf16 *__restrict__ Xout,
-            float *__restrict__ Yout, const float *__restrict__ Zprev)
-{
-    typedef ConvCfg<CIN, COUT> C;
-    typedef Mfma<COUT> M;
-    __shared__ bf16x8 wl[C::KS * 64];
-    for (int i = threadIdx.x; i < C::KS * 64; i += 64 * CONV_WAVES) wl[i] = Wp[i];
-    __syncthreads();
-
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int col = lane % C::TILE, part = lane / C::TILE;
-    const int i1 = g.in[1], i2 = g.in[2];
-    const int o0 = g.out[0], o1 = g.out[1], o2 = g.out[2];
-    const long long plane = (long long)i1 * i2;
-    const long long stride = (long long)gridDim.x * CONV_WAVES * C::WAVE_CELLS;
-    for (long long base = ((long long)blockIdx.x * CONV_WAVES + wave) * C::WAVE_CELLS; base < g.ncells;
-         base += stride) {
-        long long inb[C::NT];
-        int ox[C::NT], oy[C::NT], oz[C::NT];
-        bool ok[C::NT];
-#pragma unroll
-        for (int t = 0; t < C::NT; ++t) {
-            const long long m = base + t * C::TILE + col;
-            ok[t] = m < g.ncells;
-            const long long mm = ok[t] ? m : 0;
-            oz[t] = (int)(mm % o2);
-            const long long r = mm / o2;
-            oy[t] = (int)(r % o1);
-            const long long r2 = r / o1;
-            ox[t] = (int)(r2 % o0);
-            const long long b = r2 / o0;
-            inb[t] = ((b * g.in[0] + ox[t] - PAD) * i1 + oy[t] - PAD) * i2 + oz[t] - PAD;
-        }
-        // B operands of k-step s (this lane's 8 channels of its cells)
-        auto load_b = [&](int s, bf16x8 *bv) {
-            const int k = s * C::KSTEP + part * 8;
-            const int tap = k / CIN, c0 = k % CIN;
-            const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;
-            const long long off = dx * plane + dy * i2 + dz;
-#pragma unroll
-            for (int t = 0; t < C::NT; ++t) {
-                bool in = ok[t] && tap < 27;
-                if (PAD) {
-                    const int ix = ox[t] + dx - PAD, iy = oy[t] + dy - PAD, iz = oz[t] + dz - PAD;
-                    in = in && ix >= 0 && ix < g.in[0] && iy >= 0 && iy < i1 && iz >= 0 && iz < i2;
-                }
-                bv[t] = in ? *reinterpret_cast<const bf16x8 *>(X + (inb[t] + off) * CIN + c0) : zero8();
-            }
-        };
-        typename M::acc_t acc[C::NT];
-#pragma unroll
-        for (int t = 0; t < C::NT; ++t)
-#pragma unroll
-            for (int i = 0; i < M::NACC; ++i) acc[t][i] = 0.0f;
-
-        // software pipeline: the loads of k-step s+1 are in flight during the MFMAs of s
-        bf16x8 bcur[C::NT], bnext[C::NT];
-        load_b(0, bcur);
-        for (int s = 0; s < C::KS; ++s) {
-            if (s + 1 < C::KS) load_b(s + 1, bnext);
-            const bf16x8 a = wl[s * 64 + lane];
-#pragma unroll
-            for (int t = 0; t < C::NT; ++t) acc[t] = M::op(a, bcur[t], acc[t]);
-#pragma unroll
-            for (int t = 0; t < C::NT; ++t) bcur[t] = bnext[t];
-        }
-
-        // epilogue: lane holds output channels M::row(i, lane) of cell base + t*TILE + col,
-        // in runs of 4 consecutive channels (accumulators 4g .. 4g+3): vector stores
-#pragma unroll
-        for (int t = 0; t < C::NT; ++t) {
-            if (!ok[t]) continue;
-            const long long m = base + t * C::TILE + col;
-#pragma unroll
-            for (int gq = 0; gq < M::NACC / 4; ++gq) {
-                const int ch = M::row(4 * gq, lane);
-                float v[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) v[r] = acc[t][4 * gq + r];
-                if (EPI == EPI_BWD) {
-                    const f32x4 zp = *reinterpret_cast<const f32x4 *>(Zprev + m * COUT + ch);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) v[r] *= gelu_grad(zp[r]);
-                } else {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) v[r] += bias[ch + r];
-                }
-                if (EPI == EPI_TRAIN) {
-                    f32x4 z;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) z[r] = v[r];
-                    *reinterpret_cast<f32x4 *>(Zout + m * COUT + ch) = z;
-                }
-                if (EPI == EPI_OUT) {
-                    f32x4 y;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) y[r] = v[r];
-                    *reinterpret_cast<f32x4 *>(Yout + m * COUT + ch) = y;
-                } else {
-                    bf16x4 x;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) x[r] = (bf16)(EPI == EPI_BWD ? v[r] : gelu(v[r]));
-                    *reinterpret_cast<bf16x4 *>(Xout + m * COUT + ch) = x;
-                }
-            }
-        }
-    }
-}
 
 // ------------------------------------------------ conv (MFMA, LDS ring) --
 // The same convolutions with the activations staged in LDS.  A workgroup

@@ -1,7 +1,10 @@
 // Host execution of emulated workgroups (see hip/hip_runtime.h): the threads
 // of a workgroup are ucontext fibers on the calling thread, resumed in
-// round-robin order; __syncthreads() yields, so every fiber reaches a barrier
-// before any leaves it.
+// round-robin order.  sync() is a barrier of the calling thread's wave (the
+// lane exchanges and matrix operations), block_sync() one of the whole
+// workgroup (__syncthreads): a fiber waiting at one yields until every live
+// fiber of its wave (or block) has arrived, so waves may run different
+// instruction streams between workgroup barriers, as on the GPU.
 #include "hip/hip_runtime.h"
 
 #include <ucontext.h>
@@ -29,15 +32,62 @@ std::vector<Fiber> fibers;
 unsigned cur = 0;
 const std::function<void()> *cur_body = nullptr;
 
+void release_if_complete(unsigned t);
+
 void fiber_main()
 {
     (*cur_body)();
     fibers[cur].done = true;
+    release_if_complete(cur);   // barriers the finished thread will not reach
     swapcontext(&fibers[cur].ctx, &sched);
 }
 }  // namespace
 
-void sync() { swapcontext(&fibers[cur].ctx, &sched); }
+// per wave and for the block: live threads, arrivals at the current barrier,
+// and the barrier's generation
+unsigned wave_live[16], wave_arrived[16], block_live = 0, block_arrived = 0;
+unsigned long wave_gen[16], block_gen = 0;
+
+namespace {
+unsigned wave_of(unsigned t) { return t / 64; }
+
+void release_if_complete(unsigned t)
+{
+    const unsigned w = wave_of(t);
+    --wave_live[w];
+    --block_live;
+    if (wave_arrived[w] && wave_arrived[w] == wave_live[w]) {
+        wave_arrived[w] = 0;
+        ++wave_gen[w];
+    }
+    if (block_arrived && block_arrived == block_live) {
+        block_arrived = 0;
+        ++block_gen;
+    }
+}
+}  // namespace
+
+void sync()
+{
+    const unsigned t = cur, w = wave_of(t);
+    const unsigned long g = wave_gen[w];
+    if (++wave_arrived[w] == wave_live[w]) {
+        wave_arrived[w] = 0;
+        ++wave_gen[w];
+    }
+    while (wave_gen[w] == g) swapcontext(&fibers[t].ctx, &sched);
+}
+
+void block_sync()
+{
+    const unsigned t = cur;
+    const unsigned long g = block_gen;
+    if (++block_arrived == block_live) {
+        block_arrived = 0;
+        ++block_gen;
+    }
+    while (block_gen == g) swapcontext(&fibers[t].ctx, &sched);
+}
 
 void launch(unsigned grid, unsigned block, const std::function<void()> &body)
 {
@@ -47,6 +97,13 @@ void launch(unsigned grid, unsigned block, const std::function<void()> &body)
     cur_body = &body;
     for (unsigned b = 0; b < grid; ++b) {
         blockIdx.x = b;
+        block_arrived = 0;
+        block_live = block;
+        for (unsigned w = 0; w < 16; ++w) {
+            wave_arrived[w] = 0;
+            const unsigned lo = 64 * w;
+            wave_live[w] = lo >= block ? 0 : (block - lo < 64 ? block - lo : 64);
+        }
         for (unsigned t = 0; t < block; ++t) {
             Fiber &f = fibers[t];
             if (!f.stack) f.stack.reset(new char[kStack]);

@@ -35,7 +35,8 @@ extern emu_idx blockDim, gridDim;
 
 namespace emu {
 extern double xbuf[1024];
-void sync();
+void sync();         // barrier of the calling thread's wave
+void block_sync();   // barrier of the workgroup
 template <typename T> T exch(T v, int src_lane_delta, int mode)
 {
     const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid & ~63;
@@ -66,7 +67,7 @@ inline int exch_dpp(int old, int v, int dir)
 }
 }  // namespace emu
 
-inline void __syncthreads() { emu::sync(); }
+inline void __syncthreads() { emu::block_sync(); }
 // wave vote; every thread of the block must reach it (as every call site in
 // the kernels does)
 inline int __any(int p)
@@ -239,6 +240,7 @@ template <typename P> inline void __builtin_amdgcn_global_load_lds(const void *s
 }
 inline void __builtin_amdgcn_s_waitcnt(int) {}
 inline void __builtin_amdgcn_sched_barrier(int) {}
+inline void __builtin_amdgcn_sched_group_barrier(int, int, int) {}
 inline int atomicAdd(int *p, int v)
 {
     const int old = *p;   // fibers of one host thread: no concurrent access
