@@ -280,7 +280,7 @@ def bench_learned(args):
                    "forward_ms": {"prep": round(ms[0], 4), "conv1": round(ms[1], 4), "conv2": round(ms[2], 4),
                                   "conv3": round(ms[3], 4), "total": round(ms[4], 4)}},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": 2500.0, "unit": "TFLOP/s",
-                     "frac": round(tflops / 2500.0, 4), "traffic": None, "kernel": "conv_kernel<32, 32, 1, 0>",
+                     "frac": round(tflops / 2500.0, 4), "traffic": None, "kernel": "conv_ring_kernel<32, 32, 1, 0>",
                      "kernel_ms": round(ms[2], 4), "flop_per_launch": flops},
     }
 
