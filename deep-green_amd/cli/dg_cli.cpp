@@ -8,6 +8,7 @@
 //          [--precision fp32|fp64|fp32_bulk] [--math exact|fast]
 //          [--limiter none|minmod|vanleer] [--out DIR] [--no-dump] [--quiet]
 //          [--restart DIR/state.NNNNN]
+//   dg_cli --config sod64|blast256|khi512|ensemble|learned128 --steps N [--seed S]
 //
 // Dumps: DIR/state.NNNNN/ (main.rs:58-60) with one .npy file per dataset of
 // the reference's HDF5 layout (io.rs:5-196), float64, full ghosted shape:
@@ -22,6 +23,7 @@
 // wtime, wstep), from which --restart continues the run exactly.
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -433,6 +435,294 @@ int load_dump(dg_ctx *ctx, const Grid &shape, int precision, const std::string &
     return dg_set_state(ctx, f, DG_CONSERVED);
 }
 
+// ------------------------------------------------ ensemble and learned --
+// Seeded splitmix64 streams for the two workloads below: the same
+// distributions as the Python set-ups (deepgreen/setups.py), not the same
+// numbers.
+struct Rng {
+    uint64_t s;
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    }
+    double uni() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+    double uniform(double a, double b) { return a + (b - a) * uni(); }
+    double normal()
+    {
+        const double u1 = (double)((next() >> 11) + 1) * (1.0 / 9007199254740993.0), u2 = uni();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+    }
+};
+
+// Host primitive fields of a ghosted grid in the context's precision.
+struct HostState {
+    std::vector<std::vector<char>> buf;
+    const void *f[5];
+    HostState(const Grid &g, bool fp64) : buf(5)
+    {
+        const size_t n = (size_t)g.nx * g.ny * g.nz;
+        for (int q = 0; q < 5; ++q) {
+            buf[q].resize(n * (fp64 ? 8 : 4));
+            for (size_t i = 0; i < n; ++i) {
+                if (fp64) reinterpret_cast<double *>(buf[q].data())[i] = g.f[q][i];
+                else reinterpret_cast<float *>(buf[q].data())[i] = g.f[q][i];
+            }
+            f[q] = buf[q].data();
+        }
+    }
+};
+
+// Cell centre of ghosted index i on the unit cube with n interior cells.
+double unit_centre(int i, int n) { return (i - 2 + 0.5) / n; }
+
+// Random planar Riemann problem (the ensemble's members): a normal uniform on
+// the sphere through the cube's centre; on each side rho, p ~ U[0.1, 1] and
+// each velocity component ~ U[-1, 1].
+void init_planar_riemann(Grid &g, int n, Rng &r)
+{
+    double nrm[3] = {r.normal(), r.normal(), r.normal()};
+    const double len = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    for (double &v : nrm) v /= len;
+    float side[2][5];
+    for (auto &sd : side) {
+        sd[0] = (float)r.uniform(0.1, 1.0);
+        sd[4] = (float)r.uniform(0.1, 1.0);
+        for (int d = 1; d <= 3; ++d) sd[d] = (float)r.uniform(-1.0, 1.0);
+    }
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                const double s = (unit_centre(i, n) - 0.5) * nrm[0] + (unit_centre(j, n) - 0.5) * nrm[1] +
+                                 (unit_centre(k, n) - 0.5) * nrm[2];
+                const float *v = side[s < 0.0 ? 0 : 1];
+                for (int q = 0; q < 5; ++q) g.at(q, i, j, k) = v[q];
+            }
+}
+
+// The learned workload's initial condition on an n^3 periodic unit cube:
+// density 1 + 8 Fourier modes (integer wave vectors in [-4, 4]^3, amplitudes
+// in [-0.2, 0.2], random phases), scaled down uniformly where they would take
+// the density below 0.2; pressure 1 + 1-3 Gaussian blasts of peak 5-20 (width
+// 0.05, periodic distance); at rest.
+void init_learned(Grid &g, int n, uint64_t seed)
+{
+    Rng r{seed};
+    struct Mode { double k[3], a, ph; };
+    std::vector<Mode> modes;
+    while (modes.size() < 8) {
+        Mode m;
+        bool any = false;
+        for (double &k : m.k) {
+            k = (double)(int)(r.next() % 9) - 4.0;
+            any = any || k != 0.0;
+        }
+        if (!any) continue;
+        m.a = r.uniform(-0.2, 0.2);
+        m.ph = r.uniform(0.0, 2.0 * M_PI);
+        modes.push_back(m);
+    }
+    struct Blast { double c[3], peak; };
+    std::vector<Blast> blasts(1 + r.next() % 3);
+    for (Blast &b : blasts) {
+        for (double &c : b.c) c = r.uni();
+        b.peak = r.uniform(5.0, 20.0);
+    }
+    const double sigma = 0.05;
+    auto gauss = [&](double x, double c) {
+        double d = std::fabs(x - c);
+        d = std::min(d, 1.0 - d);
+        return std::exp(-d * d / (2.0 * sigma * sigma));
+    };
+    std::vector<double> S((size_t)g.nx * g.ny * g.nz);
+    double smin = 0.0;
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                const double x = unit_centre(i, n), y = unit_centre(j, n), z = unit_centre(k, n);
+                double sm = 0.0, p = 1.0;
+                for (const Mode &m : modes) sm += m.a * std::sin(2.0 * M_PI * (m.k[0] * x + m.k[1] * y + m.k[2] * z) + m.ph);
+                for (const Blast &b : blasts) p += (b.peak - 1.0) * gauss(x, b.c[0]) * gauss(y, b.c[1]) * gauss(z, b.c[2]);
+                const size_t c = ((size_t)i * g.ny + j) * g.nz + k;
+                S[c] = sm;
+                smin = std::min(smin, sm);
+                g.f[4][c] = (float)p;
+            }
+    const double rho_min = 0.2, scale = 1.0 + smin >= rho_min ? 1.0 : (1.0 - rho_min) / (-smin);
+    for (size_t c = 0; c < S.size(); ++c) g.f[0][c] = (float)(1.0 + scale * S[c]);
+}
+
+// The learned workload's solver: periodic, FP32 fast math, minmod, gamma 1.4, CFL 0.4.
+dg_config learned_config(int n)
+{
+    dg_config cfg;
+    dg_config_default(&cfg);
+    cfg.gamma = 1.4;
+    cfg.cfl = 0.4;
+    cfg.g = 0.0;
+    for (int d = 0; d < 3; ++d) cfg.n[d] = n;
+    for (int d = 0; d < 6; ++d) cfg.bc[d] = DG_PERIODIC;
+    cfg.ds = 1.0 / n;
+    cfg.precision = DG_FP32;
+    cfg.math = DG_MATH_FAST;
+    cfg.limiter = DG_LIMITER_MINMOD;
+    return cfg;
+}
+
+// --config ensemble: `members` cubic grids with edges drawn by seed from {32,
+// 48, 64, 96, 128}, random planar Riemann problems, outflow, gamma 1.4, CFL
+// 0.4, advanced `steps` steps in one batched run.
+int run_ensemble(uint64_t seed, int members, long long steps, int prec, int math, int limiter)
+{
+    static const int sizes[5] = {32, 48, 64, 96, 128};
+    Rng r{seed};
+    std::vector<dg_config> cfgs(members);
+    long long cells = 0;
+    for (dg_config &c : cfgs) {
+        dg_config_default(&c);
+        const int n = sizes[r.next() % 5];
+        c.gamma = 1.4;
+        c.cfl = 0.4;
+        c.g = 0.0;
+        for (int d = 0; d < 3; ++d) c.n[d] = n;
+        for (int d = 0; d < 6; ++d) c.bc[d] = DG_OUTFLOW;
+        c.ds = 1.0 / n;
+        c.precision = prec;
+        c.math = math;
+        c.limiter = limiter;
+        cells += (long long)n * n * n;
+    }
+    dg_ensemble *e = nullptr;
+    int rc = dg_ensemble_create(cfgs.data(), members, 0, &e);
+    if (rc) {
+        std::fprintf(stderr, "dg_ensemble_create failed (%d): %s\n", rc, dg_ensemble_last_error(e));
+        return 1;
+    }
+    std::printf("Running ensemble of %d grids (%lld cells).\n", members, cells);
+    for (int i = 0; i < members; ++i) {
+        const int n = cfgs[i].n[0];
+        Grid g(n, n, n, (float)cfgs[i].ds);
+        Rng ri{seed * 0x9E3779B97F4A7C15ULL + (uint64_t)i + 1};
+        init_planar_riemann(g, n, ri);
+        HostState h(g, prec == DG_FP64);
+        if ((rc = dg_ensemble_set_state(e, i, h.f, DG_PRIMITIVE))) {
+            std::fprintf(stderr, "dg_ensemble_set_state: %s\n", dg_ensemble_last_error(e));
+            return 1;
+        }
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = dg_ensemble_run(e, steps, INFINITY);
+    double t = 0, nd = 0;
+    int64_t done = 0;
+    if (!rc) rc = dg_ensemble_clock(e, 0, &t, &done, &nd);   // waits for the run
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc) {
+        std::fprintf(stderr, "dg_ensemble_run: %s\n", dg_ensemble_last_error(e));
+        return 1;
+    }
+    bool finite = true;
+    for (int i = 0; i < members; ++i) {
+        double sums[5];
+        finite = finite && dg_ensemble_conservation_sums(e, i, sums) == 0;
+        for (double v : sums) finite = finite && std::isfinite(v);
+    }
+    dg_ensemble_destroy(e);
+    std::printf("Ensemble done in %.3fs! (%lld steps, %.2f Gcell-updates/s including graph capture, state %s)\n",
+                secs, (long long)done, cells * (double)done / secs / 1e9, finite ? "finite" : "NOT FINITE");
+    return finite ? 0 : 1;
+}
+
+// --config learned128: the learned flux correction on a 128^3 grid of the
+// workload's initial condition (seed 1000 + seed).  The network's parameters
+// come from --params (float32: dg_cnn_param_count() parameters, then the 15
+// normalization values, as FluxCNN.get_params / get_normalization give
+// them), or are trained here on `train_ics` 256^3 plain runs (a sample every
+// 4 fine steps up to t = 0.03, then `train_steps` Adam steps, lr 1e-3, batch
+// 8 of 32^3 crops).
+int run_learned(uint64_t seed, long long steps, const std::string &params, int train_ics, int train_steps)
+{
+    dg_cnn *m = nullptr;
+    int rc = dg_cnn_create(0, seed, &m);
+    if (rc) { std::fprintf(stderr, "dg_cnn_create failed (%d)\n", rc); return 1; }
+    const int64_t np = dg_cnn_param_count();
+    if (!params.empty()) {
+        std::vector<float> p(np + 15);
+        FILE *fp = std::fopen(params.c_str(), "rb");
+        const bool ok = fp && std::fread(p.data(), sizeof(float), p.size(), fp) == p.size();
+        if (fp) std::fclose(fp);
+        if (!ok || dg_cnn_set_params(m, p.data()) || dg_cnn_set_normalization(m, p.data() + np)) {
+            std::fprintf(stderr, "cannot read %lld parameters and 15 normalization values from %s\n", (long long)np,
+                         params.c_str());
+            return 1;
+        }
+        std::printf("Network parameters from %s.\n", params.c_str());
+    } else if (train_ics > 0) {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int k = 1; k <= train_ics; ++k) {
+            dg_config cf = learned_config(256), cc = learned_config(128);
+            dg_ctx *fine = nullptr, *coarse = nullptr;
+            if ((rc = dg_create(&cf, &fine)) || (rc = dg_create(&cc, &coarse))) {
+                std::fprintf(stderr, "dg_create failed (%d)\n", rc);
+                return 1;
+            }
+            Grid g(256, 256, 256, (float)cf.ds);
+            init_learned(g, 256, seed + (uint64_t)k);
+            HostState h(g, false);
+            if ((rc = dg_set_state(fine, h.f, DG_PRIMITIVE))) return 1;
+            double t = 0;
+            int64_t n = 0;
+            while (t < 0.03) {
+                if ((rc = dg_cnn_add_sample(m, fine, coarse)) || (rc = dg_run(fine, 2, INFINITY, &t, &n))) {
+                    std::fprintf(stderr, "sampling failed (%d)\n", rc);
+                    return 1;
+                }
+            }
+            dg_destroy(fine);
+            dg_destroy(coarse);
+            std::printf("Training run %d of %d: %d samples.\n", k, train_ics, dg_cnn_sample_count(m));
+            std::fflush(stdout);
+        }
+        std::vector<float> losses(train_steps);
+        if ((rc = dg_cnn_fit_normalization(m)) ||
+            (rc = dg_cnn_train(m, train_steps, 8, 32, 1e-3f, seed, losses.data()))) {
+            std::fprintf(stderr, "training failed (%d)\n", rc);
+            return 1;
+        }
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("Trained on %d samples in %.1fs, loss %g -> %g.\n", dg_cnn_sample_count(m), secs,
+                    (double)losses.front(), (double)losses.back());
+    } else {
+        std::printf("Untrained network (seeded weights).\n");
+    }
+    dg_config cfg = learned_config(128);
+    dg_ctx *ctx = nullptr;
+    if ((rc = dg_create(&cfg, &ctx))) { std::fprintf(stderr, "dg_create failed (%d)\n", rc); return 1; }
+    Grid g(128, 128, 128, (float)cfg.ds);
+    init_learned(g, 128, 1000 + seed);
+    HostState h(g, false);
+    if ((rc = dg_set_state(ctx, h.f, DG_PRIMITIVE))) return 1;
+    double s0[5], s1[5];
+    dg_conservation_sums(ctx, s0);
+    double t = 0;
+    int64_t n = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = dg_learned_run(ctx, m, steps, INFINITY, &t, &n);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc) { std::fprintf(stderr, "dg_learned_run: %s\n", dg_last_error(ctx)); return 1; }
+    int64_t flagged = 0;
+    dg_learned_flagged(m, &flagged);
+    dg_conservation_sums(ctx, s1);
+    const bool finite = std::isfinite(s1[0]) && std::isfinite(s1[4]);
+    std::printf("Learned run done in %.3fs! (%lld steps, t = %g, mass drift %.2e, %lld cells on the plain flux in "
+                "the last step, state %s)\n", secs, (long long)n, t, (s1[0] - s0[0]) / s0[0], (long long)flagged,
+                finite ? "finite" : "NOT FINITE");
+    dg_destroy(ctx);
+    dg_cnn_destroy(m);
+    return finite ? 0 : 1;
+}
+
 void usage()
 {
     std::fprintf(stderr,
@@ -443,6 +733,10 @@ void usage()
                  "       dg_cli --config sod64|blast256|khi512 --steps N [--seed S] [options above]\n"
                  "              (gamma 1.4, CFL 0.4, no gravity; dumps only with --out: after the\n"
                  "              first step, then every --tout; default precision fp64, fp64, fp32)\n"
+                 "       dg_cli --config ensemble --steps N [--seed S] [--members M]\n"
+                 "              [--precision ..] [--math ..] [--limiter ..] (default fp32, fast, minmod)\n"
+                 "       dg_cli --config learned128 --steps N [--seed S] [--params FILE]\n"
+                 "              [--train-ics K] [--train-steps T] (default 32 and 2000)\n"
                  "demos: KELVIN_HELMHOLTZ THERMAL_RISING_BUBBLE THERMAL_RISING_BUBBLES EXPLOSION RELAXATION\n"
                  "       SHOCK_TUBE RAYLEIGH_TAYLOR RAYLEIGH_BERNARD\n");
 }
@@ -455,7 +749,9 @@ int main(int argc, char **argv)
     float tout = -1, total_time = -1;
     long long max_steps = -1;
     uint64_t seed = 1;
-    bool do_dump = true, quiet = false, out_given = false;
+    bool do_dump = true, quiet = false, out_given = false, math_given = false, limiter_given = false;
+    int members = 256, train_ics = 32, train_steps = 2000;
+    std::string params;
     for (int a = 1; a < argc; ++a) {
         const std::string k = argv[a];
         auto val = [&]() -> std::string {
@@ -467,8 +763,12 @@ int main(int argc, char **argv)
         else if (k == "--total-time") total_time = std::strtof(val().c_str(), nullptr);
         else if (k == "--max-steps" || k == "-m") max_steps = std::strtoll(val().c_str(), nullptr, 10);
         else if (k == "--precision") precision = val();
-        else if (k == "--math") math = val();
-        else if (k == "--limiter") limiter = val();
+        else if (k == "--math") { math = val(); math_given = true; }
+        else if (k == "--limiter") { limiter = val(); limiter_given = true; }
+        else if (k == "--members") members = std::atoi(val().c_str());
+        else if (k == "--params") params = val();
+        else if (k == "--train-ics") train_ics = std::atoi(val().c_str());
+        else if (k == "--train-steps") train_steps = std::atoi(val().c_str());
         else if (k == "--out") { out = val(); out_given = true; }
         else if (k == "--no-dump") do_dump = false;
         else if (k == "--quiet") quiet = true;
@@ -485,6 +785,15 @@ int main(int argc, char **argv)
         std::fprintf(stderr, "unknown --precision, --math or --limiter value\n");
         usage();
         return 2;
+    }
+    if (config == "ensemble" || config == "learned128") {
+        if (!demo.empty() || max_steps < 0 || members < 1 || train_ics < 0 || train_steps < 1) { usage(); return 2; }
+        if (config == "learned128") return run_learned(seed, max_steps, params, train_ics, train_steps);
+        const int prec = precision == "fp64" ? DG_FP64 : (precision == "fp32_bulk" ? DG_FP32_BULK : DG_FP32);
+        const int mth = !math_given || math == "fast" ? DG_MATH_FAST : DG_MATH_EXACT;
+        const int lim = !limiter_given || limiter == "minmod" ? DG_LIMITER_MINMOD
+                                                              : (limiter == "vanleer" ? DG_LIMITER_VANLEER : DG_LIMITER_NONE);
+        return run_ensemble(seed, members, max_steps, prec, mth, lim);
     }
     Demo d;
     std::string default_precision = "fp32";

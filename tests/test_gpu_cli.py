@@ -157,3 +157,45 @@ def test_cli_benchmark_config_khi512_runs(tmp_path):
     (fast math, as bench.py times it)."""
     out = run_cli(["--config", "khi512", "--steps", "3", "--math", "fast", "--quiet"], str(tmp_path))
     assert "Running simulation khi512: 512x512x512 cells" in out and "3 steps" in out
+
+
+def test_cli_config_ensemble_runs(tmp_path):
+    """--config ensemble: a seeded batch of planar Riemann grids (edges from
+    {32, 48, 64, 96, 128}, outflow, minmod, FP32 fast math) advanced in one
+    batched run; every member's conserved totals stay finite."""
+    out = run_cli(["--config", "ensemble", "--steps", "3", "--members", "16", "--seed", "5"], str(tmp_path))
+    assert "Running ensemble of 16 grids" in out
+    assert "(3 steps," in out and "state finite" in out
+
+
+def test_cli_config_learned128_trains_and_runs(tmp_path):
+    """--config learned128: samples from one 256^3 plain run, a short
+    training, then learned steps on the 128^3 grid: mass conserved to
+    rounding (the correction telescopes), state finite."""
+    out = run_cli(["--config", "learned128", "--steps", "4", "--train-ics", "1", "--train-steps", "20"],
+                  str(tmp_path))
+    assert "Training run 1 of 1" in out and "Trained on" in out
+    line = [ln for ln in out.splitlines() if ln.startswith("Learned run done")][0]
+    assert "(4 steps" in line and "state finite" in line
+    drift = float(line.split("mass drift ")[1].split(",")[0])
+    assert abs(drift) < 1e-6
+
+
+def test_cli_config_learned128_reads_parameters(tmp_path):
+    """--params: the network's parameters and normalization as FluxCNN
+    writes them (float32), so a network trained from Python runs from the
+    command line; a file of the wrong size is rejected."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "deep-green_amd"))
+    from deepgreen.learned import FluxCNN
+    m = FluxCNN(seed=4)
+    blob = np.concatenate([m.get_params(), m.get_normalization()]).astype(np.float32)
+    m.close()
+    path = tmp_path / "net.f32"
+    blob.tofile(path)
+    out = run_cli(["--config", "learned128", "--steps", "2", "--params", str(path)], str(tmp_path))
+    assert "Network parameters from" in out and "state finite" in out
+    blob[:-1].tofile(path)
+    r = subprocess.run([CLI, "--config", "learned128", "--steps", "2", "--params", str(path)], cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "cannot read" in r.stderr
