@@ -427,3 +427,44 @@ def test_khi512_translation_equivariance_and_conservation():
     scale = np.array([a0[0], a0[0], a0[0], a0[0], a0[4]])
     assert np.all(np.abs(a1 - a0) <= 1e-6 * scale), (a0, a1)
     assert np.all(np.isfinite(U1))
+
+
+@pytest.mark.parametrize("precision, math", [("fp64", "exact"), ("fp32_bulk", "exact"), ("fp32", "fast")])
+def test_blast256_mirror_symmetry(precision, math):
+    """The blast configuration at its size (256^3 periodic, minmod, 20
+    steps): the initial state is exactly symmetric under the reflection of
+    each axis about the centre (density and energy even, the reflected
+    momentum component odd).  The scheme's slopes and Rusanov fluxes are
+    sign-symmetric, and the exact-math update takes the x flux difference
+    first (physics.rs:539-593: (((Fx+ - Fx-) + Fy+) - Fy-) + ...), so the x
+    mirror holds bit for bit; the y and z differences are added to a running
+    sum, whose rounding is not mirror-symmetric, so those mirrors (and, in
+    fast math, whose update is reassociated into fused multiply-adds, all
+    three) hold to a few ulps of the state (FP64) or a few FP32 roundings
+    amplified by the limiter.  A mistake in an axis's slot rotation, a face
+    index or a sign breaks them by far more."""
+    from deepgreen import PERIODIC, Simulation, Solver
+    from deepgreen.setups import blast
+    n = 256
+    w, ds = blast(n, np.float64 if precision == "fp64" else np.float32)
+    sim = Simulation(n, n, n, ds, [PERIODIC] * 6, gamma=1.4, cfl=0.4, g=0.0, precision=precision, math=math,
+                     limiter="minmod")
+    with Solver(sim) as s:
+        s.set_primitives(w)
+        t, steps = s.run(20)
+        U = np.stack([np.array(s.interior(a)) for a in s.get_state()])
+    assert steps == 20 and np.isfinite(U).all()
+    assert U[4].max() > 1.001 * U[4].min()   # the blast has spread: a nontrivial state
+    tol = 1e-13 if precision == "fp64" else 1e-6   # relative L1; relative Linf within 10x
+    for axis in range(3):
+        R = np.flip(U, axis=axis + 1)
+        for q in range(5):
+            sign = -1 if q == axis + 1 else 1
+            if axis == 0 and math == "exact":
+                np.testing.assert_array_equal(U[q], sign * R[q], err_msg=f"field {q}, x mirror")
+            else:
+                d = np.abs(U[q].astype(np.float64) - sign * R[q].astype(np.float64))
+                ref = np.abs(U[q].astype(np.float64))
+                l1, linf = d.sum() / max(ref.sum(), 1e-300), d.max() / max(ref.max(), 1e-300)
+                _say(f"  field {q}, axis {axis}: rel L1 {l1:.2e}, rel Linf {linf:.2e}")
+                assert l1 <= tol and linf <= 10 * tol, (q, axis, l1, linf)
