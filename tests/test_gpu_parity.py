@@ -355,29 +355,38 @@ def _rel_l1(U, Uo):
     return np.where(ref > 0, d / np.where(ref > 0, ref, 1.0), d)
 
 
-@pytest.mark.parametrize("case", ["sod64", "blast64"])
+@pytest.mark.parametrize("case", ["sod64", "blast64", "walls48"])
 def test_fp64_fast_within_1e12_of_oracle(case):
     """FP64 fast math (the scheme reassociated around FMAs, refined hardware
     reciprocals and reciprocal square roots) against the FP64 oracle: relative
     L1 per conserved field <= 1e-12 after 100 steps of the Sod tube (64^3,
-    outflow, the benchmark list's CPU case) and of the blast (64^3, periodic,
-    minmod)."""
+    outflow, the benchmark list's CPU case), of the blast (64^3, periodic,
+    minmod) and of a smooth random state in a box with walls, outflow and
+    periodic faces, gravity and van Leer slopes (48 x 40 x 70)."""
     from deepgreen import Simulation, Solver
+    prm = dict(gamma=1.4, cfl=0.4, g=0.0)
+    gravity = False
     if case == "sod64":
         c = cases.build_case("sod64")
         w, ds, n, bcs, limiter = c["w"], c["ds"], c["interior"], c["bcs"], "none"
-    else:
+    elif case == "blast64":
         n = (64, 64, 64)
         w, ds = cases.blast(n, radius=0.1 * 64 / 64, p_in=10.0, p_out=0.1)
         bcs, limiter = [cases.PERIODIC] * 6, "minmod"
-    prm = dict(gamma=1.4, cfl=0.4, g=0.0)
+    else:
+        n = (48, 40, 70)
+        w, ds = cases.smooth_random(n, seed=11, amp_v=0.2, p_base=5.0)
+        w = [np.asarray(a, np.float64) for a in w]
+        bcs, limiter, gravity = [2, 1, 0, 0, 1, 2], "vanleer", True
+        prm = dict(gamma=5.0 / 3.0, cfl=0.3, g=-0.5)
     _say(f"  fp64 fast {case}")
-    with Solver(Simulation(*n, ds, bcs, precision="fp64", math="fast", limiter=limiter, **prm)) as s:
+    with Solver(Simulation(*n, ds, bcs, precision="fp64", math="fast", limiter=limiter, gravity=gravity,
+                           **prm)) as s:
         s.set_primitives(w)
         t, steps = s.run(100)
         assert steps == 100
         U = np.stack([np.array(s.interior(a)) for a in s.get_state()])
-    o = Oracle(n, ds, bcs, precision="f64", limiter=limiter, nthreads=8, **prm)
+    o = Oracle(n, ds, bcs, precision="f64", limiter=limiter, nthreads=8, gravity=gravity, gravity_k_shift=0, **prm)
     o.set_primitives(w)
     _, to, _ = o.compute(1e30, 100)
     Uo = np.stack([np.array(o.interior_view(a)) for a in o.cons])
