@@ -23,6 +23,8 @@ Prints one JSON line (rank 0).  Fields beyond the contract:
   cpu_baseline  the reference CPU path on this host, on a bounded sample.
   secondary     (default KHI run) short runs of the other configurations:
                 KHI 512^3 FP64 fast and FP32 exact, the ensemble, the blast.
+  pcie_inclusive  (default KHI run) the rate of a host that hands its arrays
+                over every step (set state, step, get state); never `value`.
 """
 import argparse
 import json
@@ -154,10 +156,27 @@ def bench_khi(args):
     alg_bytes = 10 * el * cells
     achieved = alg_bytes / (ms_step * 1e-3) / 1e9
     finite = bool(np.isfinite(solver.conservation_sums()).all())
+    pcie = None
+    if getattr(args, "pcie", False):
+        # a host that hands its arrays over every step (dg_set_state, dg_step,
+        # dg_get_state with the reference's ghosted layout in pageable memory)
+        U = solver.get_state()
+        reps = 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            solver.set_conserved(U)
+            solver.compute_step()
+            U = solver.get_state()
+        el_s = (time.perf_counter() - t0) / reps
+        pcie = {"value": round(cells / el_s / 1e9, 3), "unit": "Gcell-updates/s", "ms_per_step": round(el_s * 1e3, 1),
+                "note": f"per step: dg_set_state(conserved) + dg_step + dg_get_state(conserved) of ghosted "
+                        f"{n + 4}^3 arrays in pageable host memory; never the headline value"}
+        del U
     solver.close()
     workload = f"khi{n}_{args.precision}_{args.math}"
     traffic = traffic_from_profile(f"step_kernel_{workload}_hbm.md", ms_step)
     return {
+        "pcie_inclusive": pcie,
         "s_per_step": elapsed / args.steps, "value": gcells, "dtype": "f64" if args.precision == "fp64" else "f32",
         "config": {"workload": f"khi{n}_{args.precision}", "cells": cells, "periodic": True,
                    "gamma": 1.4, "cfl": 0.4, "math": args.math, "limiter": "none",
@@ -374,8 +393,8 @@ def secondary(args):
         out[name] = entry
         print(f"[bench] secondary {name}: {entry['value']} Gcell-updates/s", file=sys.stderr, flush=True)
 
-    run("khi512_fp64_fast", bench_khi, precision="fp64", math="fast", steps=48, warmup=16)
-    run("khi512_fp32_exact", bench_khi, precision="fp32", math="exact", steps=48, warmup=16)
+    run("khi512_fp64_fast", bench_khi, precision="fp64", math="fast", steps=48, warmup=16, pcie=False)
+    run("khi512_fp32_exact", bench_khi, precision="fp32", math="exact", steps=48, warmup=16, pcie=False)
     run("ensemble256_fp32_fast", bench_ensemble, workload="ensemble", math="fast", steps=40)
     run("blast256_fp64_vs_fp32bulk", bench_blast, workload="blast", steps=500)
     return out
@@ -435,6 +454,7 @@ def main():
     elif args.workload == "blast":
         r = bench_blast(args)
     else:
+        args.pcie = not args.no_secondary
         r = bench_khi(args)
     sec = None
     if args.workload == "khi" and not args.no_secondary and args.n == 512 and args.precision == "fp32" \
@@ -463,6 +483,8 @@ def main():
     }
     if sec is not None:
         out["secondary"] = sec
+    if r.get("pcie_inclusive"):
+        out["pcie_inclusive"] = r["pcie_inclusive"]
     print(json.dumps(out), flush=True)
 
 
