@@ -15,6 +15,8 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <thread>
+#include <algorithm>
 
 #include "../../include/deepgreen.h"
 #include "dg_ctx.h"
@@ -78,8 +80,30 @@ inline void host_map(int64_t gi, int64_t n, int lo, int hi, int64_t *src, int *f
     }
 }
 
+// fn(lo, hi) over [0, n) split across up to 16 host threads (the copies
+// between the caller's ghosted arrays and the pinned staging buffers)
+template <typename F> void parallel_for(int64_t n, F fn)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int64_t nt = std::min<int64_t>(std::min<int64_t>(16, hw ? hw : 1), n);
+    if (nt <= 1) {
+        fn(int64_t(0), n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int64_t chunk = (n + nt - 1) / nt;
+    for (int64_t t = 0; t < nt; ++t) {
+        const int64_t lo = t * chunk, hi = std::min(n, lo + chunk);
+        if (lo < hi) th.emplace_back([=] { fn(lo, hi); });
+    }
+    for (auto &t : th) t.join();
+}
+
+// One field of the interior state into the caller's ghosted array, ghost
+// cells by the boundary maps (velocity component f = 1, 2, 3 negated across
+// a wall of its own axis).
 template <typename T>
-void scatter_ghosted(const dg_ctx *c, const std::vector<const T *> &interior, T *const out[5])
+void scatter_ghosted(const dg_ctx *c, const T *interior, T *out, int f)
 {
     const int64_t nx = c->cfg.n[0], ny = c->cfg.n[1], nz = c->cfg.n[2];
     const int64_t gy = ny + 4, gz = nz + 4;
@@ -87,20 +111,35 @@ void scatter_ghosted(const dg_ctx *c, const std::vector<const T *> &interior, T 
     std::vector<int> fy(gy), fz(gz);
     for (int64_t j = 0; j < gy; ++j) host_map(j, ny, c->cfg.bc[2], c->cfg.bc[3], &sy[j], &fy[j]);
     for (int64_t k = 0; k < gz; ++k) host_map(k, nz, c->cfg.bc[4], c->cfg.bc[5], &sz[k], &fz[k]);
-    for (int64_t i = 0; i < nx + 4; ++i) {
-        int64_t sx; int fx;
-        host_map(i, nx, c->cfg.bc[0], c->cfg.bc[1], &sx, &fx);
-        for (int64_t j = 0; j < gy; ++j)
-            for (int64_t k = 0; k < gz; ++k) {
-                const int64_t s = (sx * ny + sy[j]) * nz + sz[k];
-                const int64_t d = (i * gy + j) * gz + k;
-                for (int f = 0; f < 5; ++f) {
-                    T v = interior[f][s];
+    parallel_for(nx + 4, [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+            int64_t sx; int fx;
+            host_map(i, nx, c->cfg.bc[0], c->cfg.bc[1], &sx, &fx);
+            for (int64_t j = 0; j < gy; ++j)
+                for (int64_t k = 0; k < gz; ++k) {
+                    T v = interior[(sx * ny + sy[j]) * nz + sz[k]];
                     if ((f == 1 && fx) || (f == 2 && fy[j]) || (f == 3 && fz[k])) v = -v;
-                    out[f][d] = v;
+                    out[(i * gy + j) * gz + k] = v;
                 }
-            }
+        }
+    });
+}
+
+// The two pinned staging buffers and their events, at the first transfer.
+int ensure_staging(dg_ctx *c)
+{
+    for (int b = 0; b < 2; ++b) {
+        if (!c->stage[b] && hipHostMalloc(&c->stage[b], c->n3 * c->el, hipHostMallocDefault) != hipSuccess) {
+            c->stage[b] = nullptr;
+            (void)hipGetLastError();
+            return fail(c, DG_ERR_ALLOC, "pinned staging buffer");
+        }
+        if (!c->stage_ev[b] && hipEventCreateWithFlags(&c->stage_ev[b], hipEventDisableTiming) != hipSuccess) {
+            c->stage_ev[b] = nullptr;
+            return fail(c, DG_ERR_DEVICE, "staging event");
+        }
     }
+    return DG_OK;
 }
 
 } // namespace
@@ -449,7 +488,10 @@ void dg_destroy(dg_ctx *c)
     if (c->scratch) (void)hipFree(c->scratch);
     if (c->dumpbuf) (void)hipFree(c->dumpbuf);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
-    if (c->stage) (void)hipHostFree(c->stage);
+    for (int b = 0; b < 2; ++b) {
+        if (c->stage[b]) (void)hipHostFree(c->stage[b]);
+        if (c->stage_ev[b]) (void)hipEventDestroy(c->stage_ev[b]);
+    }
     if (c->stream && c->owns_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -462,18 +504,23 @@ int dg_set_state(dg_ctx *c, const void *const f[5], int kind)
     HIPCHK(c, hipSetDevice(c->cfg.device));
     const int64_t nx = c->cfg.n[0], ny = c->cfg.n[1], nz = c->cfg.n[2];
     const int64_t gy = ny + 4, gz = nz + 4;
-    if (!c->stage) HIPCHK(c, hipHostMalloc(&c->stage, c->n3 * c->el, hipHostMallocDefault));
+    if (int rc = ensure_staging(c)) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // field q is packed into stage[q % 2] while field q - 1 crosses to the device
     for (int q = 0; q < 5; ++q) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));  // staging buffer reuse
+        const int b = q & 1;
+        if (q >= 2) HIPCHK(c, hipEventSynchronize(c->stage_ev[b]));   // field q - 2 has left stage[b]
         const char *src = static_cast<const char *>(f[q]);
-        char *dst = static_cast<char *>(c->stage);
-        for (int64_t i = 0; i < nx; ++i)
-            for (int64_t j = 0; j < ny; ++j) {
-                const int64_t s = ((i + 2) * gy + (j + 2)) * gz + 2;
-                const int64_t d = (i * ny + j) * nz;
-                std::memcpy(dst + d * c->el, src + s * c->el, nz * c->el);
-            }
-        HIPCHK(c, hipMemcpyAsync(c->buf[c->cur][q], c->stage, c->n3 * c->el, hipMemcpyHostToDevice, c->stream));
+        char *dst = static_cast<char *>(c->stage[b]);
+        const size_t el = c->el;
+        parallel_for(nx, [&](int64_t i0, int64_t i1) {
+            for (int64_t i = i0; i < i1; ++i)
+                for (int64_t j = 0; j < ny; ++j)
+                    std::memcpy(dst + (i * ny + j) * nz * el, src + (((i + 2) * gy + (j + 2)) * gz + 2) * el,
+                                nz * el);
+        });
+        HIPCHK(c, hipMemcpyAsync(c->buf[c->cur][q], c->stage[b], c->n3 * c->el, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipEventRecord(c->stage_ev[b], c->stream));
     }
     if (kind == DG_PRIMITIVE) {
         void *u[5];
@@ -501,20 +548,24 @@ int dg_get_state(dg_ctx *c, void *const f[5], int kind)
             return fail(c, DG_ERR_DEVICE, "cons_to_prim");
         srcs = prim;
     }
-    std::vector<char> host(5 * c->n3 * c->el);
-    for (int q = 0; q < 5; ++q)
-        HIPCHK(c, hipMemcpyAsync(host.data() + q * c->n3 * c->el, srcs[q], c->n3 * c->el, hipMemcpyDeviceToHost,
-                                 c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (c->prec == DG_FP64) {
-        std::vector<const double *> in(5);
-        for (int q = 0; q < 5; ++q) in[q] = reinterpret_cast<const double *>(host.data() + q * c->n3 * c->el);
-        scatter_ghosted<double>(c, in, reinterpret_cast<double *const *>(f));
-    } else {
-        std::vector<const float *> in(5);
-        for (int q = 0; q < 5; ++q) in[q] = reinterpret_cast<const float *>(host.data() + q * c->n3 * c->el);
-        scatter_ghosted<float>(c, in, reinterpret_cast<float *const *>(f));
+    if (int rc = ensure_staging(c)) return rc;
+    // field q crosses into stage[q % 2] while field q - 1 is scattered on the host
+    auto scatter = [&](int q) {
+        if (c->prec == DG_FP64)
+            scatter_ghosted<double>(c, static_cast<const double *>(c->stage[q & 1]), static_cast<double *>(f[q]), q);
+        else
+            scatter_ghosted<float>(c, static_cast<const float *>(c->stage[q & 1]), static_cast<float *>(f[q]), q);
+    };
+    for (int q = 0; q < 5; ++q) {
+        HIPCHK(c, hipMemcpyAsync(c->stage[q & 1], srcs[q], c->n3 * c->el, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipEventRecord(c->stage_ev[q & 1], c->stream));
+        if (q >= 1) {
+            HIPCHK(c, hipEventSynchronize(c->stage_ev[(q - 1) & 1]));
+            scatter(q - 1);
+        }
     }
+    HIPCHK(c, hipEventSynchronize(c->stage_ev[0]));
+    scatter(4);
     return DG_OK;
 }
 

@@ -24,7 +24,8 @@ struct dg_ctx {
     int nparts_reduce = 0;
     dg::Scalars *sc = nullptr;      // device
     dg::Scalars *sc_host = nullptr; // pinned mirror
-    void *stage = nullptr;          // pinned staging (one field), allocated at first upload
+    void *stage[2] = {nullptr, nullptr};     // pinned staging, one field each (double-buffered host transfers)
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};  // copy out of / into stage[b] done
     void *scratch = nullptr;        // device scratch (one state) for primitive downloads
     void *dumpbuf = nullptr;        // ghosted arrays of dg_step_dump (allocated at the first dump)
     hipStream_t stream = nullptr;

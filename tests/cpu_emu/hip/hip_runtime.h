@@ -106,7 +106,7 @@ typedef void *hipGraphExec_t;
 enum { hipSuccess = 0, hipErrorUnknown = 999 };
 enum { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3, hipStreamNonBlocking = 1,
        hipHostMallocDefault = 0, hipStreamCaptureModeThreadLocal = 0,
-       hipDeviceAttributeMultiprocessorCount = 16 };
+       hipDeviceAttributeMultiprocessorCount = 16, hipEventDisableTiming = 2 };
 typedef int hipMemcpyKind;
 
 namespace emu {
@@ -151,6 +151,7 @@ inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, int) { *s = (void *)1
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventCreate(hipEvent_t *e) { *e = (void *)1; return hipSuccess; }
+inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { *e = (void *)1; return hipSuccess; }
 inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
