@@ -278,9 +278,9 @@ conv_ring_kernel(const RingGeom rg, const bf16 *__restrict__ X, const bf16x8 *__
         }
         __syncthreads();
 
-        // The epilogue of plane x runs in slices inside the k-loop of plane x + 1
-        // (its VALU work overlaps the MFMAs; one wave per SIMD has no other
-        // wave to hide it behind); the last plane's runs after the loop.
+        // The epilogue of plane x runs in slices between the k-steps of plane
+        // x + 1 (no store waits at the plane's end); the last plane's runs
+        // after the loop.
         constexpr int NEPI = NT * (M::NACC / 4);
         auto epilogue = [&](const typename M::acc_t *A, long long mb, int e) {
             const int t = e / (M::NACC / 4), gq = e % (M::NACC / 4);
