@@ -490,20 +490,29 @@ dw_kernel(const ConvGeom g, const bf16 *__restrict__ X, const bf16 *__restrict__
     }
 }
 
-// grad[tap][cin_real][cout_real] (+)= sum over workgroups of the partials.
+// grad[tap][cin_real][cout_real] = sum over workgroups of the partials:
+// one block per (tap, cin); 8 groups of 32 lanes (one per output channel)
+// each sum every 8th workgroup in order, then the 8 group sums are added in
+// order (a fixed order: deterministic).
 __global__ void __launch_bounds__(256)
 dw_reduce_kernel(const float *__restrict__ partials, int nwg, int cin_pad, int cin_real, int cout_real,
                  float *__restrict__ grad)
 {
+    __shared__ double red[8][32];
     const int ncol = 27 * cin_pad;
     const int ntiles = (ncol + 31) / 32;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= 27 * cin_real * cout_real) return;
-    const int co = e % cout_real, ci = (e / cout_real) % cin_real, tap = e / (cout_real * cin_real);
+    const int tap = blockIdx.x / cin_real, ci = blockIdx.x % cin_real;
     const int n = tap * cin_pad + ci;
+    const int co = threadIdx.x & 31, grp = threadIdx.x >> 5;
     double s = 0.0;
-    for (int w = 0; w < nwg; ++w) s += (double)partials[((size_t)w * ntiles * 32 + n) * 32 + co];
-    grad[e] = (float)s;
+    for (int w = grp; w < nwg; w += 8) s += (double)partials[((size_t)w * ntiles * 32 + n) * 32 + co];
+    red[grp][co] = s;
+    __syncthreads();
+    if (grp == 0 && co < cout_real) {
+        double t = 0.0;
+        for (int q = 0; q < 8; ++q) t += red[q][co];
+        grad[(size_t)blockIdx.x * cout_real + co] = (float)t;
+    }
 }
 
 // bias gradient: db[c] = sum over cells of dZ[cell][c] (partials per block)
@@ -524,13 +533,22 @@ db_kernel(const bf16 *__restrict__ dZ, long long ncells, int cpad, float *__rest
     }
 }
 
-__global__ void db_reduce_kernel(const float *__restrict__ partials, int nblk, int creal, float *__restrict__ grad)
+// grad[c] = sum over blocks of the partials: 8 groups of 32 channels each sum
+// every 8th block in order, then the group sums are added in order.
+__global__ void __launch_bounds__(256)
+db_reduce_kernel(const float *__restrict__ partials, int nblk, int creal, float *__restrict__ grad)
 {
-    const int c = threadIdx.x;
-    if (c >= creal) return;
+    __shared__ double red[8][32];
+    const int c = threadIdx.x & 31, grp = threadIdx.x >> 5;
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += (double)partials[b * 32 + c];
-    grad[c] = (float)s;
+    for (int b = grp; b < nblk; b += 8) s += (double)partials[b * 32 + c];
+    red[grp][c] = s;
+    __syncthreads();
+    if (grp == 0 && c < creal) {
+        double t = 0.0;
+        for (int q = 0; q < 8; ++q) t += red[q][c];
+        grad[c] = (float)t;
+    }
 }
 
 // ---------------------------------------------------- weights, Adam --
@@ -827,8 +845,7 @@ int dw(int layer, int B, const int in[3], const void *X, const void *dZ, float *
 int dw_reduce(int layer, const float *partials, int nwg, float *grad, hipStream_t s)
 {
     const int cin_pad = layer == 1 ? 8 : 32, cin_real = layer == 1 ? 5 : 32, cout_real = layer == 3 ? 15 : 32;
-    const int n = 27 * cin_real * cout_real;
-    hipLaunchKernelGGL(dw_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, s, partials, nwg, cin_pad, cin_real,
+    hipLaunchKernelGGL(dw_reduce_kernel, dim3(27 * cin_real), dim3(256), 0, s, partials, nwg, cin_pad, cin_real,
                        cout_real, grad);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -836,7 +853,7 @@ int dw_reduce(int layer, const float *partials, int nwg, float *grad, hipStream_
 int db(const void *dZ, long long ncells, int cpad, int creal, float *partials, int nblk, float *grad, hipStream_t s)
 {
     hipLaunchKernelGGL(db_kernel, dim3(nblk), dim3(256), 0, s, static_cast<const bf16 *>(dZ), ncells, cpad, partials);
-    hipLaunchKernelGGL(db_reduce_kernel, dim3(1), dim3(32), 0, s, partials, nblk, creal, grad);
+    hipLaunchKernelGGL(db_reduce_kernel, dim3(1), dim3(256), 0, s, partials, nblk, creal, grad);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
