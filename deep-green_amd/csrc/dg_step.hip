@@ -1172,14 +1172,15 @@ __device__ __forceinline__ void corr_delta(const StepArgs &A, const float *__res
 __global__ void zero_count_kernel(int *cnt) { *cnt = 0; }
 
 // MASKED = false: flags[i] = corrected state of i invalid with every face
-// corrected, and cnt counts the flagged cells.  MASKED = true (skipped when
-// cnt is 0): flags[i] = prev[i] or state of i invalid with the faces of the
-// cells flagged in prev kept plain.
+// corrected, cnt counts the flagged cells, and D (5 fields of n3) keeps the
+// increments with every face corrected for the apply kernel.  MASKED = true
+// (skipped when cnt is 0): flags[i] = prev[i] or state of i invalid with the
+// faces of the cells flagged in prev kept plain.
 template <bool FAST, bool MASKED>
 __global__ void __launch_bounds__(256)
 corr_flag_kernel(const StepArgs A, const Fields<const float> U, const float *__restrict__ G, const CorrScale S,
                  const Scalars *__restrict__ sc, const unsigned char *__restrict__ prev,
-                 unsigned char *__restrict__ flags, int *__restrict__ cnt)
+                 unsigned char *__restrict__ flags, int *__restrict__ cnt, float *__restrict__ D)
 {
     if (MASKED && *cnt == 0) return;
     const float ds = float(A.ds), V = ds * ds * ds, gm1 = float(A.gamma) - 1.0f;
@@ -1197,25 +1198,35 @@ corr_flag_kernel(const StepArgs A, const Fields<const float> U, const float *__r
         } else {
             flags[i] = bad ? 1 : 0;
             if (bad) atomicAdd(cnt, 1);
+            for (int q = 0; q < 5; ++q) D[q * n3 + i] = d[q];
         }
     }
 }
 
+// With no cell flagged (cnt 0) every face is corrected and the increments
+// are the ones the flag kernel kept in D (the same arithmetic: masks of 1).
 template <bool FAST>
 __global__ void __launch_bounds__(256)
 corr_apply_kernel(const StepArgs A, const Fields<float> U, const float *__restrict__ G, const CorrScale S,
-                  const Scalars *__restrict__ sc, const unsigned char *__restrict__ flags, double *__restrict__ partials)
+                  const Scalars *__restrict__ sc, const unsigned char *__restrict__ flags,
+                  const int *__restrict__ cnt, const float *__restrict__ D, double *__restrict__ partials)
 {
     __shared__ double red[4 * 8];
     const float ds = float(A.ds), V = ds * ds * ds, gm1 = float(A.gamma) - 1.0f, gamma = float(A.gamma);
     const float coef = sc->active ? -float(sc->dt) * ds * ds : 0.0f;
     const long long n3 = (long long)A.n[0] * A.n[1] * A.n[2];
+    const bool none = *cnt == 0;
     float maxden = 0.0f;
     int nanf = 0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long long)gridDim.x * blockDim.x) {
-        const int z = (int)(i % A.n[2]), y = (int)((i / A.n[2]) % A.n[1]), x = (int)(i / ((long long)A.n[1] * A.n[2]));
         float d[5], u[5], w[5];
-        corr_delta(A, G, S, coef, flags, x, y, z, d);
+        if (none) {
+            for (int q = 0; q < 5; ++q) d[q] = D[q * n3 + i];
+        } else {
+            const int z = (int)(i % A.n[2]), y = (int)((i / A.n[2]) % A.n[1]);
+            const int x = (int)(i / ((long long)A.n[1] * A.n[2]));
+            corr_delta(A, G, S, coef, flags, x, y, z, d);
+        }
         for (int q = 0; q < 5; ++q) {
             u[q] = U.f[q][i] + d[q];
             U.f[q][i] = u[q];
@@ -1512,7 +1523,10 @@ static int cells_grid(long long n3)
     return (int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
 }
 
-size_t corr_flags_bytes(long long n3) { return (size_t)((2 * n3 + 63) / 64) * 64 + 64; }
+// flags: the final flags in [0, n3), the second pass's in [n3, 2 n3), then
+// the increments D (5 n3 floats), then the count
+static size_t corr_d_offset(long long n3) { return (size_t)((2 * n3 + 63) / 64) * 64; }
+size_t corr_flags_bytes(long long n3) { return corr_d_offset(n3) + (size_t)n3 * 5 * sizeof(float) + 64; }
 
 int launch_corr(int fast, const StepArgs &a, void *const u[5], const float *G, const float scale[5], const Scalars *sc,
                 unsigned char *flags, double *partials, hipStream_t s)
@@ -1521,21 +1535,25 @@ int launch_corr(int fast, const StepArgs &a, void *const u[5], const float *G, c
     for (int q = 0; q < 5; ++q) S.s[q] = scale[q];
     const int nb = reduce_blocks(a);
     const long long n3 = (long long)a.n[0] * a.n[1] * a.n[2];
-    // flags: the final flags in [0, n3), the second pass's in [n3, 2 n3), then the count
     unsigned char *fa = flags, *fb = flags + n3;
+    float *D = reinterpret_cast<float *>(flags + corr_d_offset(n3));
     int *cnt = reinterpret_cast<int *>(flags + corr_flags_bytes(n3) - 64);
     const dim3 g(cells_grid(n3)), b(256);
     hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, s, cnt);
     if (fast) {
-        hipLaunchKernelGGL((corr_flag_kernel<true, false>), g, b, 0, s, a, cfields<float>(u), G, S, sc, nullptr, fa, cnt);
-        hipLaunchKernelGGL((corr_flag_kernel<true, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fa, fb, cnt);
-        hipLaunchKernelGGL((corr_flag_kernel<true, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fb, fa, cnt);
-        hipLaunchKernelGGL((corr_apply_kernel<true>), dim3(nb), b, 0, s, a, mfields<float>(u), G, S, sc, fa, partials);
+        hipLaunchKernelGGL((corr_flag_kernel<true, false>), g, b, 0, s, a, cfields<float>(u), G, S, sc, nullptr, fa, cnt,
+                           D);
+        hipLaunchKernelGGL((corr_flag_kernel<true, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fa, fb, cnt, D);
+        hipLaunchKernelGGL((corr_flag_kernel<true, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fb, fa, cnt, D);
+        hipLaunchKernelGGL((corr_apply_kernel<true>), dim3(nb), b, 0, s, a, mfields<float>(u), G, S, sc, fa, cnt, D,
+                           partials);
     } else {
-        hipLaunchKernelGGL((corr_flag_kernel<false, false>), g, b, 0, s, a, cfields<float>(u), G, S, sc, nullptr, fa, cnt);
-        hipLaunchKernelGGL((corr_flag_kernel<false, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fa, fb, cnt);
-        hipLaunchKernelGGL((corr_flag_kernel<false, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fb, fa, cnt);
-        hipLaunchKernelGGL((corr_apply_kernel<false>), dim3(nb), b, 0, s, a, mfields<float>(u), G, S, sc, fa, partials);
+        hipLaunchKernelGGL((corr_flag_kernel<false, false>), g, b, 0, s, a, cfields<float>(u), G, S, sc, nullptr, fa, cnt,
+                           D);
+        hipLaunchKernelGGL((corr_flag_kernel<false, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fa, fb, cnt, D);
+        hipLaunchKernelGGL((corr_flag_kernel<false, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fb, fa, cnt, D);
+        hipLaunchKernelGGL((corr_apply_kernel<false>), dim3(nb), b, 0, s, a, mfields<float>(u), G, S, sc, fa, cnt, D,
+                           partials);
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }

@@ -200,6 +200,33 @@ def test_learned_correction_fallback_and_conservation(emu_lib):
     a.close(), p.close()
 
 
+def test_learned_correction_without_flags(emu_lib):
+    """A small host-given correction that flags no cell: every face is
+    corrected (the apply pass uses the increments the flag pass kept), twice
+    in a row so the second step reads increments of its own step."""
+    from deepgreen.learned import FluxCNN
+    m = FluxCNN(seed=6, lib=emu_lib)
+    n, ds = 6, 1.0 / 6
+    a, p = periodic_solver(emu_lib, n, 2), periodic_solver(emu_lib, n, 2)
+    sums0 = a.conservation_sums()
+    rng = np.random.default_rng(11)
+    scale = m.get_normalization()[10:15]
+    for k in range(2):
+        p.set_conserved(a.get_state())
+        G = (0.05 * rng.normal(size=(n, n, n, 16))).astype(np.float32)
+        G[..., 15] = 0.0
+        dt = m.step_given(a, G)
+        assert dt == p.compute_step()
+        Up = interior(p.get_state()).astype(np.float64)
+        expect, flag = correction_reference(Up, G, dt, ds, scale, 1.4)
+        assert not flag.any() and m.flagged() == 0
+        got = interior(a.get_state())
+        assert not np.array_equal(got, Up.astype(np.float32))
+        np.testing.assert_allclose(got, expect, rtol=2e-6, atol=2e-6 * np.abs(expect).max())
+    np.testing.assert_allclose(a.conservation_sums(), sums0, rtol=0, atol=1e-5 * np.abs(Up).sum(axis=(1, 2, 3)).max())
+    a.close(), p.close()
+
+
 def test_learned_correction_fallback_reaches_the_neighbours(emu_lib):
     """Two huge mass fluxes in a row along x: cell (2, 3, 1) empties through
     its +x face and is flagged; cell (3, 3, 1) passes the mass on through its
