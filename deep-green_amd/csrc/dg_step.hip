@@ -196,6 +196,25 @@ __device__ __forceinline__ T wave_den(const T w[5], T gamma)
     }
 }
 
+// The same from a conserved state u (extensive).  Fast math: with
+// x = gamma (gamma - 1) (E - |m|^2 / (2 U_rho)), gamma p / rho = x / U_rho, so
+// the sound speed is |x| / sqrt(x U_rho) (one reciprocal square root) and
+// |v| = sqrt(|m|^2) / |U_rho|.
+template <typename T, bool FAST>
+__device__ __forceinline__ T cell_den(const T u[5], T V, T invV, T gm1, T gamma, bool pow2 = false)
+{
+    if constexpr (!FAST) {
+        T w[5];
+        to_prim<T, false>(u, V, invV, gm1, w, pow2);
+        return wave_den<T, false>(w, gamma);
+    } else {
+        const T im = drcp<T>(u[0]);
+        const T m2 = dfma<T>(u[3], u[3], dfma<T>(u[2], u[2], u[1] * u[1]));
+        const T x = (gamma * gm1) * dfma<T>(T(-0.5) * m2, im, u[4]);
+        return dfma<T>(dabs<T>(x), drsq<T>(x * u[0]), dsqrt_fast<T>(m2) * dabs<T>(im));
+    }
+}
+
 // Limited slope (extension; the reference's limiters are dead code,
 // physics.f90:205-238): textbook minmod and van Leer.
 template <typename T>
@@ -209,10 +228,12 @@ __device__ __forceinline__ T limited(T a, T b, int lim)
 // Rusanov flux of calc_flux_reconstruction (physics.rs:338-405) for one face.
 // l = state of the cell on the + side ("l" in the reference), r = state of the
 // cell on the - side, both (rho, v_normal, v_t1, v_t2, p); F returns the
-// rotated fluxes multiplied by ds (FAST: not multiplied by ds).
+// rotated fluxes multiplied by ds.  FAST: the fifth component of l and r is
+// P = 2 p / (gamma - 1) (so 2 E = rho |v|^2 + P), F is not multiplied by ds,
+// hgm1 = (gamma - 1) / 2 and sK = sqrt(gamma (gamma - 1) / 2).
 template <typename T, bool FAST>
-__device__ __forceinline__ void rusanov(const T l[5], const T r[5], T F[5], T gamma, T gm1, T igm1, T ds,
-                                        bool grav, T G, T h)
+__device__ __forceinline__ void rusanov(const T l[5], const T r[5], T F[5], T gamma, T gm1, T ds, bool grav, T G,
+                                        T h, T hgm1, T sK)
 {
     const T rl = l[0], vxl = l[1], vyl = l[2], vzl = l[3], pl = l[4];
     const T rr = r[0], vxr = r[1], vyr = r[2], vzr = r[3], pr = r[4];
@@ -244,39 +265,41 @@ __device__ __forceinline__ void rusanov(const T l[5], const T r[5], T F[5], T ga
         F[0] = f0 * ds; F[1] = f1 * ds; F[2] = f2 * ds; F[3] = f3 * ds; F[4] = f4 * ds;
     } else {
         // sums S = 2 * (star state); fluxes returned as 2 * flux / 1 (the
-        // caller folds the factor 1/2 and ds into the update).
+        // caller folds the factor 1/2 and ds into the update).  pl, pr hold
+        // P = 2 p / (gamma - 1): the doubled total energies are
+        // e2 = m . v + P, and Se2 = 2 Se.
+        (void)gamma; (void)gm1;
         const T mxl = rl * vxl, myl = rl * vyl, mzl = rl * vzl;
         const T mxr = rr * vxr, myr = rr * vyr, mzr = rr * vzr;
-        const T el = dfma<T>(T(0.5), dfma<T>(mzl, vzl, dfma<T>(myl, vyl, mxl * vxl)), pl * igm1);
-        const T er = dfma<T>(T(0.5), dfma<T>(mzr, vzr, dfma<T>(myr, vyr, mxr * vxr)), pr * igm1);
-        const T Sr = rl + rr, Sx = mxl + mxr, Sy = myl + myr, Sz = mzl + mzr, Se = el + er;
+        const T e2l = dfma<T>(mzl, vzl, dfma<T>(myl, vyl, dfma<T>(mxl, vxl, pl)));
+        const T e2r = dfma<T>(mzr, vzr, dfma<T>(myr, vyr, dfma<T>(mxr, vxr, pr)));
+        const T Sr = rl + rr, Sx = mxl + mxr, Sy = myl + myr, Sz = mzl + mzr, Se2 = e2l + e2r;
         const T iS = drcp<T>(Sr);
         const T m2 = dfma<T>(Sz, Sz, dfma<T>(Sy, Sy, Sx * Sx));
-        const T P2 = gm1 * dfma<T>(T(-0.5) * m2, iS, Se);   // 2 p*
+        const T P2 = hgm1 * dfma<T>(-m2, iS, Se2);          // 2 p* = (gamma - 1) (Se - m2 / (2 Sr))
         const T un = Sx * iS;                                // u* normal
         T f0 = Sx;
         T f1 = dfma<T>(Sx, un, P2);
         T f2 = Sy * un;
         T f3 = Sz * un;
-        T f4 = (Se + P2) * un;
+        T f4 = dfma<T>(T(0.5), Se2, P2) * un;
         if (grav) {
             const T gh = G * h;
             f1 = dfma<T>(Sr, gh, f1);
             f4 = dfma<T>(T(0.5) * Sr * (vxl + vxr), gh, f4);
         }
-        // sqrt(gamma p / rho) = |gamma p| / sqrt(gamma p rho): one reciprocal
-        // square root; the absolute value keeps the identity for the negative
-        // face pressures and densities the unlimited scheme produces at strong
+        // sqrt(gamma p / rho) = sK |P| / sqrt(P rho): one reciprocal square
+        // root; the absolute value keeps the identity for the negative face
+        // pressures and densities the unlimited scheme produces at strong
         // jumps (p / rho > 0), and a negative p / rho gives NaN as sqrt does
-        const T gpl = gamma * pl, gpr = gamma * pr;
-        const T al = dfma<T>(dabs<T>(gpl), drsq<T>(gpl * rl), dabs<T>(vxl));
-        const T ar = dfma<T>(dabs<T>(gpr), drsq<T>(gpr * rr), dabs<T>(vxr));
+        const T al = dfma<T>(sK * dabs<T>(pl), drsq<T>(pl * rl), dabs<T>(vxl));
+        const T ar = dfma<T>(sK * dabs<T>(pr), drsq<T>(pr * rr), dabs<T>(vxr));
         const T c = dmax<T>(al, ar);
         F[0] = dfma<T>(-c, rl - rr, f0);
         F[1] = dfma<T>(-c, mxl - mxr, f1);
         F[2] = dfma<T>(-c, myl - myr, f2);
         F[3] = dfma<T>(-c, mzl - mzr, f3);
-        F[4] = dfma<T>(-c, el - er, f4);
+        F[4] = dfma<T>(T(-0.5) * c, e2l - e2r, f4);
         (void)ds;
     }
 }
@@ -292,7 +315,8 @@ __device__ __forceinline__ int rot_field(int axis, int slot)
 
 template <typename T>
 struct Consts {
-    T ds, gamma, gm1, igm1, V, invV, two_ds, inv_two_ds, inv_ds, half_ds, half_dt, dtds, G;
+    T ds, gamma, gm1, V, invV, two_ds, inv_two_ds, inv_ds, half_ds, half_dt, dtds, G;
+    T hgm1, sK, irk;   // fast math: (gamma - 1) / 2, sqrt(gamma (gamma - 1) / 2), (gamma - 1) V / 2
     bool pow2;   // ds a power of two: divisions by ds, 2 ds, ds^3 as exact products
 };
 
@@ -304,7 +328,6 @@ __device__ __forceinline__ Consts<T> make_consts(const StepArgs &A, T dt)
     C.ds = T(A.ds);
     C.gamma = T(A.gamma);
     C.gm1 = C.gamma - T(1.0);
-    C.igm1 = T(1) / C.gm1;
     C.V = C.ds * C.ds * C.ds;
     C.invV = T(1) / C.V;
     C.two_ds = T(2.0) * C.ds;
@@ -315,6 +338,9 @@ __device__ __forceinline__ Consts<T> make_consts(const StepArgs &A, T dt)
     C.G = T(A.g);
     C.half_dt = FAST ? T(0.5) * dt * C.inv_two_ds : T(0.5) * dt;
     C.dtds = FAST ? dt * C.ds * C.ds * T(0.5) : dt * C.ds;
+    C.hgm1 = T(0.5) * C.gm1;
+    C.sK = dsqrt<T>(C.gamma * C.hgm1);
+    C.irk = C.hgm1 * C.V;
     return C;
 }
 
@@ -324,14 +350,15 @@ template <typename T> using vec2 = T __attribute__((ext_vector_type(2)));
 // LDS layout of the step kernel.
 template <typename T, int TY>
 struct StepLDS {
-    // The primitives ring is cell-major: a cell's 5 primitives and (fast
-    // math) its inverse density sit in 6 consecutive words, read as three
+    // The primitives ring is cell-major: a cell's 6 words (ring_cell: the
+    // 5 primitives and, in fast math, its scaled inverse density, the
+    // pressure word holding 2 p / (gamma - 1)) are consecutive, read as three
     // pairs; every neighbour of a cell, in all three ring slots, is then
     // within the DS offset range of one address register.
     static constexpr int NC = (TY + 4) * RZ;  // region cells of one plane
     static constexpr int CS = 6;              // words per cell
     static constexpr int PS = NC * CS;        // one plane
-    alignas(16) T W[3 * PS];  // [plane slot][row][col][rho, vx, vy, vz, p, 1/rho]
+    alignas(16) T W[3 * PS];  // [plane slot][row][col][ring_cell words]
     // Ym[.][r]: -y face state of row r (r = 0..TY) until the flux phase, in
     // which the one thread that reads it replaces it by the flux through
     // face (r-1, r) (r = 0: the tile's lower y face)
@@ -341,6 +368,29 @@ struct StepLDS {
     T Zm[5][TY];             // -z face state of column 64
     T Z0[5][TY];             // -z face state of column 0, then the flux through the tile's lower z face
 };
+
+// The six ring words of a cell from its conserved state u.  Exact math:
+// calc_primitives (rho, vx, vy, vz, p) and a pad.  Fast math: the pressure
+// word holds P = 2 p / (gamma - 1) = (2 E - |m|^2 / U_rho) / V, which the
+// flux uses as is (twice the internal energy density) and the Hancock step
+// extrapolates like p (it is linear in p), and the sixth word
+// 1 / (k rho) = irk / U_rho with k = 2 / (gamma - 1), so that the Hancock
+// step's grad(p) / rho is that word times grad(P).
+template <typename T, bool FAST>
+__device__ __forceinline__ void ring_cell(const T u[5], const Consts<T> &C, T w[6])
+{
+    if constexpr (!FAST) {
+        to_prim<T, false>(u, C.V, C.invV, C.gm1, w, C.pow2);
+        w[5] = T(0);
+    } else {
+        const T im = drcp<T>(u[0]);
+        const T ux = u[1] * im, uy = u[2] * im, uz = u[3] * im;
+        w[0] = u[0] * C.invV; w[1] = ux; w[2] = uy; w[3] = uz;
+        const T ke2 = dfma<T>(uz, u[3], dfma<T>(uy, u[2], ux * u[1]));   // |m|^2 / U_rho
+        w[4] = dfma<T>(T(2), u[4], -ke2) * C.invV;
+        w[5] = C.irk * im;
+    }
+}
 
 // Slope of one field along one axis from the primitives m (at -1), c, p (at
 // +1).  Exact math: the gradient of calc_gradients (physics.rs:206-208) or
@@ -400,7 +450,8 @@ __device__ __forceinline__ void ld_prims(const T *p, T v[6])
 
 // Hancock half-step (extrapolate_primitives_t, physics.rs:217-282) of a
 // cell with primitives w, slopes g (g[f*3+d]: field f along axis d) and, in
-// fast math, inverse density ir.
+// fast math, the ring's sixth word ir (w[4] then holds P = 2 p / (gamma - 1),
+// which the step extrapolates like p, and ir grad P = grad p / rho).
 template <typename T, bool FAST>
 __device__ __forceinline__ void hancock(const T w[5], T ir, const T g[15], const Consts<T> &C, T xtr[5])
 {
@@ -501,7 +552,7 @@ __device__ __forceinline__ void face_flux(const T l[5], const T r[5], int axis, 
     T lr[5], rr[5], F[5];
 #pragma unroll
     for (int s = 0; s < 5; ++s) { lr[s] = l[rot_field(axis, s)]; rr[s] = r[rot_field(axis, s)]; }
-    rusanov<T, FAST>(lr, rr, F, C.gamma, C.gm1, C.igm1, C.ds, grav, C.G, h);
+    rusanov<T, FAST>(lr, rr, F, C.gamma, C.gm1, C.ds, grav, C.G, h, C.hgm1, C.sK);
 #pragma unroll
     for (int s = 0; s < 5; ++s) out[rot_field(axis, s)] = F[s];
 }
@@ -592,9 +643,7 @@ __device__ __forceinline__ void finish_plane(StepLDS<T, TY> &L, int slot, const 
         const bool flips = __any(Q.xflip || (P.lds[q] >= 0 && (P.lds[q] & (3 << 13)) != 0));
         if (P.lds[q] < 0) continue;
         T w[6];
-        to_prim<T, FAST>(Q.u[q], C.V, C.invV, C.gm1, w, C.pow2);
-        // fast math: 1/rho = V / U_rho for the Hancock step; exact math pads
-        w[5] = FAST ? C.V * drcp<T>(Q.u[q][0]) : T(0);
+        ring_cell<T, FAST>(Q.u[q], C, w);
         if (flips) {
             if (Q.xflip) w[1] = -w[1];
             if (P.lds[q] & (1 << 13)) w[2] = -w[2];
@@ -871,16 +920,16 @@ __device__ __forceinline__ void step_tile(StepLDS<T, TY> &L, double *red, const 
                 un[f] = FAST ? dfma<T>(-C.dtds, acc, u) : u - C.dtds * acc;
                 at(Vout.f[f], uidx) = un[f];
             }
-            T w[5];
-            to_prim<T, FAST>(un, C.V, C.invV, C.gm1, w, C.pow2);
             if constexpr (BULK) {
+                T w[5];
+                to_prim<T, FAST>(un, C.V, C.invV, C.gm1, w, C.pow2);
                 double wd[5];
                 for (int f = 0; f < 5; ++f) wd[f] = (double)w[f];
                 const double den = wave_den<double, false>(wd, (double)C.gamma);
                 nanf |= den != den;
                 dmaxden = fmax(dmaxden, den);   // NaN-ignoring
             } else {
-                const T den = wave_den<T, FAST>(w, C.gamma);
+                const T den = cell_den<T, FAST>(un, C.V, C.invV, C.gm1, C.gamma, C.pow2);
                 nanf |= den != den;
                 maxden = dmax<T>(maxden, den);  // NaN-ignoring
             }
@@ -991,16 +1040,18 @@ reduce_kernel(const StepArgs A, const Fields<const T> U, double *__restrict__ pa
     T maxden = T(0);
     int nanf = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
-        T u[5], w[5];
+        T u[5];
         for (int f = 0; f < 5; ++f) { u[f] = U.f[f][i]; sums[f] += (double)u[f]; }
-        to_prim<T, FAST>(u, V, T(1) / V, gm1, w);   // same math as the step epilogue
+        // same math as the step epilogue
         if constexpr (BULK) {
+            T w[5];
+            to_prim<T, FAST>(u, V, T(1) / V, gm1, w);
             double wd[5];
             for (int f = 0; f < 5; ++f) wd[f] = (double)w[f];
             const double den = wave_den<double, false>(wd, (double)gamma);
             if (den != den) nanf = 1; else dmaxden = fmax(dmaxden, den);
         } else {
-            const T den = wave_den<T, FAST>(w, gamma);
+            const T den = cell_den<T, FAST>(u, V, T(1) / V, gm1, gamma);
             if (den != den) nanf = 1; else maxden = dmax<T>(maxden, den);
         }
     }
@@ -1219,7 +1270,7 @@ corr_apply_kernel(const StepArgs A, const Fields<float> U, const float *__restri
     float maxden = 0.0f;
     int nanf = 0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long long)gridDim.x * blockDim.x) {
-        float d[5], u[5], w[5];
+        float d[5], u[5];
         if (none) {
             for (int q = 0; q < 5; ++q) d[q] = D[q * n3 + i];
         } else {
@@ -1231,8 +1282,7 @@ corr_apply_kernel(const StepArgs A, const Fields<float> U, const float *__restri
             u[q] = U.f[q][i] + d[q];
             U.f[q][i] = u[q];
         }
-        to_prim<float, FAST>(u, V, 1.0f / V, gm1, w);
-        const float den = wave_den<float, FAST>(w, gamma);
+        const float den = cell_den<float, FAST>(u, V, 1.0f / V, gm1, gamma);
         if (den != den) nanf = 1; else maxden = fmaxf(maxden, den);
     }
     const double nosums[5] = {0, 0, 0, 0, 0};
@@ -1299,8 +1349,7 @@ __device__ __forceinline__ void cell_xtr(const StepArgs &A, const Fields<const T
         T u[5];
 #pragma unroll
         for (int f = 0; f < 5; ++f) u[f] = at(U.f[f], idx);
-        to_prim<T, FAST>(u, C.V, C.invV, C.gm1, w[q], C.pow2);
-        w[q][5] = FAST ? C.V * drcp<T>(u[0]) : T(0);
+        ring_cell<T, FAST>(u, C, w[q]);
         if (mx.flip) w[q][1] = -w[q][1];
         if (my.flip) w[q][2] = -w[q][2];
         if (mz.flip) w[q][3] = -w[q][3];
@@ -1345,7 +1394,8 @@ heat_kernel(const StepArgs A, const Fields<const T> Uold, const Fields<T> Unew, 
         if (A.limiter == LIM_MINMOD) cell_xtr<T, FAST, LIM_MINMOD>(A, Uold, C, i, j, k, xtr);
         else if (A.limiter == LIM_VANLEER) cell_xtr<T, FAST, LIM_VANLEER>(A, Uold, C, i, j, k, xtr);
         else cell_xtr<T, FAST, LIM_NONE>(A, Uold, C, i, j, k, xtr);
-        const T t_in = xtr[4] / (T(H.rs) * xtr[0]);
+        const T p_xtr = FAST ? xtr[4] * C.hgm1 : xtr[4];   // fast math: the ring's 2 p / (gamma - 1)
+        const T t_in = p_xtr / (T(H.rs) * xtr[0]);
         const T Q = (T(H.t_out) - t_in) * (S / (dk + T(1.0) / hc));
         const T dE = dt * ds * ((Q / V) * ds);
         T E = Unew.f[4][c];
