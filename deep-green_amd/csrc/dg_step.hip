@@ -187,13 +187,8 @@ __device__ __forceinline__ void to_prim(const T u[5], T V, T invV, T gm1, T w[5]
 template <typename T, bool FAST>
 __device__ __forceinline__ T wave_den(const T w[5], T gamma)
 {
-    if constexpr (!FAST) {
-        return dsqrt<T>(gamma * w[4] / w[0]) + dsqrt<T>(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
-    } else {
-        const T v2 = dfma<T>(w[3], w[3], dfma<T>(w[2], w[2], w[1] * w[1]));
-        const T gp = gamma * w[4];
-        return dfma<T>(dabs<T>(gp), drsq<T>(gp * w[0]), dsqrt_fast<T>(v2));   // sqrt(gamma p / rho) + |v|
-    }
+    static_assert(!FAST, "fast math takes the wave speed from the conserved state (cell_den)");
+    return dsqrt<T>(gamma * w[4] / w[0]) + dsqrt<T>(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
 }
 
 // The same from a conserved state u (extensive).  Fast math: with
