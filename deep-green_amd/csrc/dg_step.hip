@@ -1217,26 +1217,35 @@ __device__ __forceinline__ void corr_delta(const StepArgs &A, const float *__res
 
 __global__ void zero_count_kernel(int *cnt) { *cnt = 0; }
 
-// MASKED = false: flags[i] = corrected state of i invalid with every face
-// corrected, cnt counts the flagged cells, and D (5 fields of n3) keeps the
-// increments with every face corrected for the apply kernel.  MASKED = true
-// (skipped when cnt is 0): flags[i] = prev[i] or state of i invalid with the
-// faces of the cells flagged in prev kept plain.
+// MASKED = false: every cell of U gets its increment with every face
+// corrected, in place, and B (5 fields of n3) keeps the plain state;
+// flags[i] = corrected state of i invalid, cnt counts the flagged cells, and
+// each block writes the wave-speed partial of the corrected state (the
+// final one when no cell is flagged).  MASKED = true (skipped when cnt is
+// 0): flags[i] = prev[i] or the plain state B of i plus its increment with
+// the faces of the cells flagged in prev kept plain is invalid.
 template <bool FAST, bool MASKED>
 __global__ void __launch_bounds__(256)
-corr_flag_kernel(const StepArgs A, const Fields<const float> U, const float *__restrict__ G, const CorrScale S,
-                 const Scalars *__restrict__ sc, const unsigned char *__restrict__ prev,
-                 unsigned char *__restrict__ flags, int *__restrict__ cnt, float *__restrict__ D)
+corr_flag_kernel(const StepArgs A, const Fields<float> U, float *__restrict__ B, const float *__restrict__ G,
+                 const CorrScale S, const Scalars *__restrict__ sc, const unsigned char *__restrict__ prev,
+                 unsigned char *__restrict__ flags, int *__restrict__ cnt, double *__restrict__ partials)
 {
     if (MASKED && *cnt == 0) return;
-    const float ds = float(A.ds), V = ds * ds * ds, gm1 = float(A.gamma) - 1.0f;
+    __shared__ double red[4 * 8];
+    const float ds = float(A.ds), V = ds * ds * ds, gm1 = float(A.gamma) - 1.0f, gamma = float(A.gamma);
     const float coef = sc->active ? -float(sc->dt) * ds * ds : 0.0f;   // an inactive step changes nothing
     const long long n3 = (long long)A.n[0] * A.n[1] * A.n[2];
+    float maxden = 0.0f;
+    int nanf = 0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long long)gridDim.x * blockDim.x) {
         const int z = (int)(i % A.n[2]), y = (int)((i / A.n[2]) % A.n[1]), x = (int)(i / ((long long)A.n[1] * A.n[2]));
         float d[5], u[5], w[5];
         corr_delta(A, G, S, coef, MASKED ? prev : nullptr, x, y, z, d);
-        for (int q = 0; q < 5; ++q) u[q] = U.f[q][i] + d[q];
+        for (int q = 0; q < 5; ++q) {
+            const float up = MASKED ? B[q * n3 + i] : U.f[q][i];
+            if (!MASKED) B[q * n3 + i] = up;
+            u[q] = up + d[q];
+        }
         to_prim<float, FAST>(u, V, 1.0f / V, gm1, w);
         const bool bad = !(w[0] > 0.0f && w[4] > 0.0f);
         if (MASKED) {
@@ -1244,37 +1253,39 @@ corr_flag_kernel(const StepArgs A, const Fields<const float> U, const float *__r
         } else {
             flags[i] = bad ? 1 : 0;
             if (bad) atomicAdd(cnt, 1);
-            for (int q = 0; q < 5; ++q) D[q * n3 + i] = d[q];
+            for (int q = 0; q < 5; ++q) U.f[q][i] = u[q];
+            const float den = cell_den<float, FAST>(u, V, 1.0f / V, gm1, gamma);
+            if (den != den) nanf = 1; else maxden = fmaxf(maxden, den);
         }
+    }
+    if (!MASKED) {
+        const double nosums[5] = {0, 0, 0, 0, 0};
+        block_partials<256>((double)maxden, nanf, nosums, red, partials, blockIdx.x, threadIdx.x);
     }
 }
 
-// With no cell flagged (cnt 0) every face is corrected and the increments
-// are the ones the flag kernel kept in D (the same arithmetic: masks of 1).
+// Only when a cell is flagged (cnt > 0): U = B + the increments under the
+// final flags, and the wave-speed partials of that state.  With no cell
+// flagged the flag kernel's corrected state and partials stand.
 template <bool FAST>
 __global__ void __launch_bounds__(256)
-corr_apply_kernel(const StepArgs A, const Fields<float> U, const float *__restrict__ G, const CorrScale S,
-                  const Scalars *__restrict__ sc, const unsigned char *__restrict__ flags,
-                  const int *__restrict__ cnt, const float *__restrict__ D, double *__restrict__ partials)
+corr_apply_kernel(const StepArgs A, const Fields<float> U, const float *__restrict__ B, const float *__restrict__ G,
+                  const CorrScale S, const Scalars *__restrict__ sc, const unsigned char *__restrict__ flags,
+                  const int *__restrict__ cnt, double *__restrict__ partials)
 {
+    if (*cnt == 0) return;
     __shared__ double red[4 * 8];
     const float ds = float(A.ds), V = ds * ds * ds, gm1 = float(A.gamma) - 1.0f, gamma = float(A.gamma);
     const float coef = sc->active ? -float(sc->dt) * ds * ds : 0.0f;
     const long long n3 = (long long)A.n[0] * A.n[1] * A.n[2];
-    const bool none = *cnt == 0;
     float maxden = 0.0f;
     int nanf = 0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(i % A.n[2]), y = (int)((i / A.n[2]) % A.n[1]), x = (int)(i / ((long long)A.n[1] * A.n[2]));
         float d[5], u[5];
-        if (none) {
-            for (int q = 0; q < 5; ++q) d[q] = D[q * n3 + i];
-        } else {
-            const int z = (int)(i % A.n[2]), y = (int)((i / A.n[2]) % A.n[1]);
-            const int x = (int)(i / ((long long)A.n[1] * A.n[2]));
-            corr_delta(A, G, S, coef, flags, x, y, z, d);
-        }
+        corr_delta(A, G, S, coef, flags, x, y, z, d);
         for (int q = 0; q < 5; ++q) {
-            u[q] = U.f[q][i] + d[q];
+            u[q] = B[q * n3 + i] + d[q];
             U.f[q][i] = u[q];
         }
         const float den = cell_den<float, FAST>(u, V, 1.0f / V, gm1, gamma);
@@ -1569,36 +1580,35 @@ static int cells_grid(long long n3)
 }
 
 // flags: the final flags in [0, n3), the second pass's in [n3, 2 n3), then
-// the increments D (5 n3 floats), then the count
-static size_t corr_d_offset(long long n3) { return (size_t)((2 * n3 + 63) / 64) * 64; }
-size_t corr_flags_bytes(long long n3) { return corr_d_offset(n3) + (size_t)n3 * 5 * sizeof(float) + 64; }
+// the plain state B (5 n3 floats), then the count
+static size_t corr_b_offset(long long n3) { return (size_t)((2 * n3 + 63) / 64) * 64; }
+size_t corr_flags_bytes(long long n3) { return corr_b_offset(n3) + (size_t)n3 * 5 * sizeof(float) + 64; }
 
 int launch_corr(int fast, const StepArgs &a, void *const u[5], const float *G, const float scale[5], const Scalars *sc,
                 unsigned char *flags, double *partials, hipStream_t s)
 {
     CorrScale S;
     for (int q = 0; q < 5; ++q) S.s[q] = scale[q];
-    const int nb = reduce_blocks(a);
+    const int nb = reduce_blocks(a);   // the flag and apply kernels write the finalize's partials
     const long long n3 = (long long)a.n[0] * a.n[1] * a.n[2];
     unsigned char *fa = flags, *fb = flags + n3;
-    float *D = reinterpret_cast<float *>(flags + corr_d_offset(n3));
+    float *B = reinterpret_cast<float *>(flags + corr_b_offset(n3));
     int *cnt = reinterpret_cast<int *>(flags + corr_flags_bytes(n3) - 64);
+    const Fields<float> U = mfields<float>(u);
     const dim3 g(cells_grid(n3)), b(256);
     hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, s, cnt);
     if (fast) {
-        hipLaunchKernelGGL((corr_flag_kernel<true, false>), g, b, 0, s, a, cfields<float>(u), G, S, sc, nullptr, fa, cnt,
-                           D);
-        hipLaunchKernelGGL((corr_flag_kernel<true, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fa, fb, cnt, D);
-        hipLaunchKernelGGL((corr_flag_kernel<true, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fb, fa, cnt, D);
-        hipLaunchKernelGGL((corr_apply_kernel<true>), dim3(nb), b, 0, s, a, mfields<float>(u), G, S, sc, fa, cnt, D,
+        hipLaunchKernelGGL((corr_flag_kernel<true, false>), dim3(nb), b, 0, s, a, U, B, G, S, sc, nullptr, fa, cnt,
                            partials);
+        hipLaunchKernelGGL((corr_flag_kernel<true, true>), g, b, 0, s, a, U, B, G, S, sc, fa, fb, cnt, partials);
+        hipLaunchKernelGGL((corr_flag_kernel<true, true>), g, b, 0, s, a, U, B, G, S, sc, fb, fa, cnt, partials);
+        hipLaunchKernelGGL((corr_apply_kernel<true>), dim3(nb), b, 0, s, a, U, B, G, S, sc, fa, cnt, partials);
     } else {
-        hipLaunchKernelGGL((corr_flag_kernel<false, false>), g, b, 0, s, a, cfields<float>(u), G, S, sc, nullptr, fa, cnt,
-                           D);
-        hipLaunchKernelGGL((corr_flag_kernel<false, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fa, fb, cnt, D);
-        hipLaunchKernelGGL((corr_flag_kernel<false, true>), g, b, 0, s, a, cfields<float>(u), G, S, sc, fb, fa, cnt, D);
-        hipLaunchKernelGGL((corr_apply_kernel<false>), dim3(nb), b, 0, s, a, mfields<float>(u), G, S, sc, fa, cnt, D,
+        hipLaunchKernelGGL((corr_flag_kernel<false, false>), dim3(nb), b, 0, s, a, U, B, G, S, sc, nullptr, fa, cnt,
                            partials);
+        hipLaunchKernelGGL((corr_flag_kernel<false, true>), g, b, 0, s, a, U, B, G, S, sc, fa, fb, cnt, partials);
+        hipLaunchKernelGGL((corr_flag_kernel<false, true>), g, b, 0, s, a, U, B, G, S, sc, fb, fa, cnt, partials);
+        hipLaunchKernelGGL((corr_apply_kernel<false>), dim3(nb), b, 0, s, a, U, B, G, S, sc, fa, cnt, partials);
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
