@@ -399,6 +399,8 @@ __device__ __forceinline__ T slope(T m, T c, T p, const Consts<T> &C)
         else return p - m;
     } else {
         if constexpr (!FAST) return C.pow2 ? limited<T>(c - m, p - c, LIM) * C.inv_ds : limited<T>(c - m, p - c, LIM) / C.ds;
+        else if constexpr (LIM == LIM_MINMOD && sizeof(T) == sizeof(float))
+            return T(2) * __builtin_amdgcn_fmed3f(c - m, p - c, 0.0f);   // minmod(a, b) = median(a, b, 0)
         else return T(2) * limited<T>(c - m, p - c, LIM);
     }
 }

@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <functional>
 #include <tuple>
 #include <vector>
@@ -91,6 +92,10 @@ inline float __builtin_amdgcn_sqrtf(float x) { return std::sqrt(x); }
 inline float __builtin_amdgcn_rsqf(float x) { return 1.0f / std::sqrt(x); }
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
 inline double __builtin_amdgcn_rsq(double x) { return 1.0 / std::sqrt(x); }
+inline float __builtin_amdgcn_fmed3f(float a, float b, float c)   // median of three (ordered inputs)
+{
+    return std::max(std::min(a, b), std::min(std::max(a, b), c));
+}
 // DPP wave_shl:1 (0x130: lane i <- lane i+1) and wave_shr:1 (0x138: lane i <- lane i-1);
 // lanes without a source keep `old`.
 inline int __builtin_amdgcn_update_dpp(int old, int v, int ctrl, int, int, bool)

@@ -79,6 +79,20 @@ def test_emulated_fast_math_close_to_exact(emu_lib):
     np.testing.assert_allclose(U, Uo, rtol=2e-5, atol=1e-6 * np.abs(Uo).max())
 
 
+def test_emulated_fast_minmod_close_to_exact(emu_lib):
+    """Fast math with minmod slopes (the median-of-three form) on a state
+    with jumps, where the limiter clips: within rounding of the oracle's
+    minmod step."""
+    interior = (7, 6, 8)
+    w, ds = cases.sod(interior, axis=1)
+    w = [np.asarray(a, np.float32) + np.float32(0.05) * np.asarray(b, np.float32)
+         for a, b in zip(w, cases.smooth_random(interior, seed=9)[0])]
+    U, dts, Uo, dto = run_both(emu_lib, interior, w, ds, [cases.OUTFLOW] * 6, 3, "fp32", limiter="minmod",
+                               math="fast")
+    np.testing.assert_allclose(dts, dto.astype(np.float64), rtol=2e-5)
+    np.testing.assert_allclose(U, Uo, rtol=2e-5, atol=1e-6 * np.abs(Uo).max())
+
+
 def test_emulated_periodic_gravity(emu_lib):
     """Periodic z with gravity: the lower boundary z-flux is the copy of the
     face (n-1, n), whose gravity height belongs to z = n-1."""

@@ -133,3 +133,23 @@ def test_kinked_profile_through_calc_gradients(prec, lim, axis):
                 go = np.asarray(o.grad[f])[..., other]
                 inner = go[tuple(slice(1, -1) for _ in range(3))]
                 assert np.all(inner == 0)
+
+
+def test_minmod_is_the_median_with_zero(fixture):
+    """The fast FP32 step evaluates minmod as the median of (a, b, 0) (one
+    hardware median instruction): equal to the oracle's minmod on every
+    fixture pair, ties, zeros and sign changes included, except where the
+    oracle's sign test a * b > 0 underflows in FP32 (|a b| below the smallest
+    subnormal: the oracle gives 0, the median the smaller of two tiny
+    same-sign differences); and different from the median with the wrong
+    third operand."""
+    a, b = fixture["a"].astype(np.float32), fixture["b"].astype(np.float32)
+    med = np.median(np.stack([a, b, np.zeros_like(a)]), axis=0).astype(np.float32)
+    ours = dg_oracle.limited(a, b, "minmod", "f32")
+    under = (a * b == 0) & (a != 0) & (b != 0)
+    assert 0 < under.sum() < 5
+    np.testing.assert_array_equal(med[~under], ours[~under])
+    assert np.all(ours[under] == 0) and np.all(np.abs(med[under]) < 1e-19)
+    assert np.any(np.abs(a) == np.abs(b)) and np.any(a * b < 0) and np.any(a == 0)
+    wrong = np.median(np.stack([a, b, np.ones_like(a)]), axis=0)
+    assert np.any(wrong != ours)
