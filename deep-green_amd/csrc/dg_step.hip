@@ -329,9 +329,12 @@ __device__ __forceinline__ Consts<T> make_consts(const StepArgs &A, T dt)
     C.inv_two_ds = T(1) / C.two_ds;
     C.inv_ds = T(1) / C.ds;
     C.pow2 = A.pow2 != 0;
-    C.half_ds = FAST ? T(0.25) : C.ds / T(2.);   // fast: slopes are scaled by 2 ds
+    // fast math: unlimited slopes are central differences (2 ds times the
+    // gradient), limited ones one-sided (ds times the gradient)
+    const bool one_sided = FAST && A.limiter != LIM_NONE;
+    C.half_ds = FAST ? (one_sided ? T(0.5) : T(0.25)) : C.ds / T(2.);
     C.G = T(A.g);
-    C.half_dt = FAST ? T(0.5) * dt * C.inv_two_ds : T(0.5) * dt;
+    C.half_dt = FAST ? T(0.5) * dt * (one_sided ? C.inv_ds : C.inv_two_ds) : T(0.5) * dt;
     C.dtds = FAST ? dt * C.ds * C.ds * T(0.5) : dt * C.ds;
     C.hgm1 = T(0.5) * C.gm1;
     C.sK = dsqrt<T>(C.gamma * C.hgm1);
@@ -389,8 +392,8 @@ __device__ __forceinline__ void ring_cell(const T u[5], const Consts<T> &C, T w[
 
 // Slope of one field along one axis from the primitives m (at -1), c, p (at
 // +1).  Exact math: the gradient of calc_gradients (physics.rs:206-208) or
-// the limited slope / ds.  Fast math: the same scaled by 2 ds (the factor is
-// folded into the constants that consume it).
+// the limited slope / ds.  Fast math: the same scaled by 2 ds (unlimited) or
+// ds (limited), the factor folded into the constants that consume it.
 template <typename T, bool FAST, int LIM>
 __device__ __forceinline__ T slope(T m, T c, T p, const Consts<T> &C)
 {
@@ -400,8 +403,8 @@ __device__ __forceinline__ T slope(T m, T c, T p, const Consts<T> &C)
     } else {
         if constexpr (!FAST) return C.pow2 ? limited<T>(c - m, p - c, LIM) * C.inv_ds : limited<T>(c - m, p - c, LIM) / C.ds;
         else if constexpr (LIM == LIM_MINMOD && sizeof(T) == sizeof(float))
-            return T(2) * __builtin_amdgcn_fmed3f(c - m, p - c, 0.0f);   // minmod(a, b) = median(a, b, 0)
-        else return T(2) * limited<T>(c - m, p - c, LIM);
+            return __builtin_amdgcn_fmed3f(c - m, p - c, 0.0f);   // minmod(a, b) = median(a, b, 0)
+        else return limited<T>(c - m, p - c, LIM);
     }
 }
 
@@ -468,7 +471,7 @@ __device__ __forceinline__ void hancock(const T w[5], T ir, const T g[15], const
         xtr[3] = uz - hd * (ux * dzx + uy * dzy + uz * dzz + (T(1.0) / r) * dpz);
         xtr[4] = p - hd * (C.gamma * p * (dxx + dyy + dzz) + ux * dpx + uy * dpy + uz * dpz);
     } else {
-        // hd here is dt / 2 / (2 ds): slopes are undivided differences
+        // hd here is dt / 2 over the slopes' scale (2 ds or ds): slopes are undivided differences
         const T div = dxx + dyy + dzz;
         xtr[0] = dfma<T>(-hd, dfma<T>(uz, drz, dfma<T>(uy, dry, dfma<T>(ux, drx, r * div))), r);
         xtr[1] = dfma<T>(-hd, dfma<T>(ir, dpx, dfma<T>(uz, dxz, dfma<T>(uy, dxy, ux * dxx))), ux);
