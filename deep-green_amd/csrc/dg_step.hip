@@ -1226,9 +1226,10 @@ __global__ void zero_count_kernel(int *cnt) { *cnt = 0; }
 // corrected, in place, and B (5 fields of n3) keeps the plain state;
 // flags[i] = corrected state of i invalid, cnt counts the flagged cells, and
 // each block writes the wave-speed partial of the corrected state (the
-// final one when no cell is flagged).  MASKED = true (skipped when cnt is
-// 0): flags[i] = prev[i] or the plain state B of i plus its increment with
-// the faces of the cells flagged in prev kept plain is invalid.
+// final one when no cell is flagged; launched with reduce_blocks blocks).
+// MASKED = true (skipped when cnt is 0; U and partials untouched):
+// flags[i] = prev[i] or the plain state B of i plus its increment with the
+// faces of the cells flagged in prev kept plain is invalid.
 template <bool FAST, bool MASKED>
 __global__ void __launch_bounds__(256)
 corr_flag_kernel(const StepArgs A, const Fields<float> U, float *__restrict__ B, const float *__restrict__ G,
