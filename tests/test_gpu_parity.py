@@ -275,6 +275,34 @@ def test_conducting_walls_match_oracle(precision):
     np.testing.assert_array_equal(U, np.stack([np.array(o.interior_view(a)) for a in o.cons]))
 
 
+@pytest.mark.parametrize("precision, tol", [("fp32", 2e-5), ("fp64", 1e-12)])
+def test_conducting_walls_fast_math_close_to_oracle(precision, tol):
+    """Fast math with conducting walls (the wall temperature from the fast
+    Hancock state, whose pressure word holds 2 p / (gamma - 1)), in a gas hot
+    enough (p / (R rho) near half of T_out) that a wrong scale would flip the
+    heat flux: relative L1 per field within rounding of the oracle after 20
+    steps."""
+    from deepgreen import Simulation, Solver
+    interior = (20, 18, 70)
+    w, ds = cases.smooth_random(interior, seed=34, amp_v=0.1, p_base=4.0e4)
+    bcs = [3, 2, 3, 1, 2, 2]
+    prm = dict(gamma=5.0 / 3.0, cfl=0.2, g=0.0)
+    _say(f"  conducting walls, fast math {precision}")
+    with Solver(Simulation(*interior, ds, bcs, precision=precision, math="fast", **prm)) as s:
+        s.set_heat(300.0, 4.0, 287.0)
+        s.set_primitives(w)
+        s.run(20)
+        U = np.stack([np.array(s.interior(a)) for a in s.get_state()])
+    o = Oracle(interior, ds, bcs, precision="f64" if precision == "fp64" else "f32", nthreads=8, t_out=300.0,
+               v_out=4.0, rs=287.0, **prm)
+    o.set_primitives(w)
+    o.compute(1e30, 20)
+    Uo = np.stack([np.array(o.interior_view(a)) for a in o.cons])
+    rel = _rel_l1(U, Uo)
+    _say(f"  rel L1 {rel}")
+    assert np.all(rel <= tol), rel
+
+
 @pytest.mark.parametrize("axis", [0, 1, 2])
 def test_conducting_wall_heat_from_extrapolated_state(axis):
     """A gas whose extrapolated state differs from its start-of-step state at

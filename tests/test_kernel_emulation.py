@@ -212,6 +212,44 @@ def test_emulated_conducting_walls_match_oracle(emu_lib, precision):
     np.testing.assert_array_equal(U, np.stack([np.array(o.interior_view(a)) for a in o.cons]))
 
 
+@pytest.mark.parametrize("precision, tol", [("fp32", 2e-5), ("fp64", 1e-12)])
+def test_emulated_fast_conducting_walls_close_to_oracle(emu_lib, precision, tol):
+    """Fast math with conducting walls: the heat source's wall temperature
+    comes from the fast Hancock state, whose pressure word holds
+    2 p / (gamma - 1).  The gas is hot enough (p / (R rho) near half of
+    T_out) that a wrong scale of that word would flip the sign of the heat
+    flux; the energy stays within rounding of the oracle's."""
+    import deepgreen
+    interior = (6, 5, 7)
+    w, ds = cases.smooth_random(interior, seed=33, amp_v=0.1, p_base=4.0e4)
+    bcs = [3, 2, 2, 1, 3, 2]
+    prm = dict(gamma=5.0 / 3.0, cfl=0.2, g=0.0)
+    sim = deepgreen.Simulation(*interior, ds, bcs, precision=precision, math="fast", tile_y=4, **prm)
+    with deepgreen.Solver(sim, lib=emu_lib) as s:
+        s.set_heat(300.0, 4.0, 287.0)
+        s.set_primitives(w)
+        s.run(6)
+        U = np.stack([np.array(s.interior(a)) for a in s.get_state()])
+    o = Oracle(interior, ds, bcs, precision="f64" if precision == "fp64" else "f32", t_out=300.0, v_out=4.0,
+               rs=287.0, **prm)
+    o.set_primitives(w)
+    o.compute(1e30, 6)
+    Uo = np.stack([np.array(o.interior_view(a)) for a in o.cons]).astype(np.float64)
+    p_over_rho = np.asarray(w[4], np.float64)[2:-2, 2:-2, 2:-2] / np.asarray(w[0], np.float64)[2:-2, 2:-2, 2:-2]
+    assert 0.3 < float(np.median(p_over_rho)) / 287.0 / 300.0 < 0.7
+    rel = np.abs(U - Uo).reshape(5, -1).sum(1) / np.abs(Uo).reshape(5, -1).sum(1)
+    assert np.all(rel <= tol), rel
+    # the heat source is visible at this tolerance: the same run with plain
+    # walls differs by over 10x tol in energy, and a heat flux of the wrong
+    # sign would be off by twice that
+    o2 = Oracle(interior, ds, [2 if b == 3 else b for b in bcs], precision="f64" if precision == "fp64" else "f32",
+                **prm)
+    o2.set_primitives(w)
+    o2.compute(1e30, 6)
+    U2 = np.stack([np.array(o2.interior_view(a)) for a in o2.cons]).astype(np.float64)
+    assert np.abs(U2 - Uo)[4].sum() / np.abs(Uo[4]).sum() > 10 * tol
+
+
 def test_emulated_noalloc_step_matches_dg_step(emu_lib):
     """dg_step_noalloc (static scratch array, caller-owned buffers) gives the
     state and dt of dg_step bit for bit, on two calls in a row, the second
