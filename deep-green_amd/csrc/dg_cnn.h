@@ -31,10 +31,11 @@ struct PrepArgs {
     unsigned int seed;              // noise stream
 };
 
-enum { CONV_TRAIN = 0, CONV_INFER = 1, CONV_OUT = 2, CONV_BWD = 3 };
+enum { CONV_TRAIN = 0, CONV_INFER = 1, CONV_OUT = 2, CONV_BWD = 3, CONV_OUT16 = 4 };
 
 // Forward (valid) convolution of `layer` (CONV_TRAIN: Z fp32 + X = gelu(Z) bf16;
-// CONV_INFER: X only; layer 3: Y fp32 16 channels), or the input gradient
+// CONV_INFER: X only; layer 3: Y fp32 16 channels, or with CONV_OUT16 X bf16
+// 16 channels), or the input gradient
 // (CONV_BWD, full convolution with the flipped weights, times gelu'(Zprev)).
 int conv(int layer, int mode, int B, const int in[3], const void *X, const void *Wp, const float *bias, float *Z,
          void *Xout, float *Y, const float *Zprev, hipStream_t s);

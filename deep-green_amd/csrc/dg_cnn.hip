@@ -125,8 +125,9 @@ template <int CIN, int COUT> struct ConvCfg {
 
 // ---------------------------------------------------------- conv (MFMA) --
 // EPI_TRAIN: Z = acc + b (fp32 out), X = bf16(gelu(Z));  EPI_INFER: X only;
-// EPI_OUT: Y = acc + b (fp32, COUT channels);  EPI_BWD: X = bf16(acc * gelu'(Zprev)).
-enum { EPI_TRAIN = 0, EPI_INFER = 1, EPI_OUT = 2, EPI_BWD = 3 };
+// EPI_OUT: Y = acc + b (fp32, COUT channels);  EPI_BWD: X = bf16(acc * gelu'(Zprev));
+// EPI_OUT16: X = bf16(acc + b) (the output layer for the learned step's correction).
+enum { EPI_TRAIN = 0, EPI_INFER = 1, EPI_OUT = 2, EPI_BWD = 3, EPI_OUT16 = 4 };
 
 // ------------------------------------------------ conv (MFMA, LDS ring) --
 // The same convolutions with the activations staged in LDS.  A workgroup
@@ -312,7 +313,7 @@ conv_ring_kernel(const RingGeom rg, const bf16 *__restrict__ X, const bf16x8 *__
             } else {
                 bf16x4 xo;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) xo[q] = (bf16)(EPI == EPI_BWD ? v[q] : gelu(v[q]));
+                for (int q = 0; q < 4; ++q) xo[q] = (bf16)(EPI == EPI_BWD || EPI == EPI_OUT16 ? v[q] : gelu(v[q]));
                 *reinterpret_cast<bf16x4 *>(Xout + m * COUT + ch) = xo;
             }
         };
@@ -817,6 +818,7 @@ int conv(int layer, int mode, int B, const int in[3], const void *X, const void 
         else DG_CONV(32, 32, EPI_BWD, 2);   // input gradient of layer 2 (dZ2 -> dZ1)
     } else if (layer == 3) {
         if (mode == CONV_BWD) DG_CONV(16, 32, EPI_BWD, 2);   // input gradient of layer 3 (dG -> dZ2)
+        else if (mode == CONV_OUT16) DG_CONV(32, 16, EPI_OUT16, 0);
         else DG_CONV(32, 16, EPI_OUT, 0);
     } else {
         return -1;

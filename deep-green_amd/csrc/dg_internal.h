@@ -100,11 +100,12 @@ int launch_ens_finalize(int prec, const EnsInstance *inst, int ninst, const doub
                         hipStream_t s);
 // Learned flux correction (FP32 periodic grids): flags cells whose corrected
 // state is not positive, applies the masked correction in place and writes
-// reduce_blocks(a) wave-speed partials for the finalize (mode 1).  flags has
+// reduce_blocks(a) wave-speed partials for the finalize (mode 1).  G: n3 x 16
+// channels, fp32 (g_bf16 = 0) or bf16 (g_bf16 = 1).  flags has
 // corr_flags_bytes(n3) bytes; its first n3 hold the final flags.
 size_t corr_flags_bytes(long long n3);
-int launch_corr(int fast, const StepArgs &a, void *const u[5], const float *G, const float scale[5], const Scalars *sc,
-                unsigned char *flags, double *partials, hipStream_t s);
+int launch_corr(int fast, const StepArgs &a, void *const u[5], const void *G, int g_bf16, const float scale[5],
+                const Scalars *sc, unsigned char *flags, double *partials, hipStream_t s);
 // FP32: coarse = 2x restriction of fine (sum of the 8 children).
 int launch_restrict(const StepArgs &fine, const void *const f[5], const StepArgs &coarse, void *const c[5], hipStream_t s);
 // FP32: r = (ut - up) / (dt ds^2).

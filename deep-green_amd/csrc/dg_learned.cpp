@@ -258,8 +258,9 @@ dg::cnn::PrepArgs prep_args(const dg_cnn *m, int B, const int n[3], int box, int
     return a;
 }
 
-// Network forward on the current state of a periodic coarse context -> G (n^3 x 16, WS_IG).
-int infer(dg_cnn *m, dg_ctx *c)
+// Network forward on the current state of a periodic coarse context -> G
+// (n^3 x 16, WS_IG): fp32, or bf16 (half) for the learned step's correction.
+int infer(dg_cnn *m, dg_ctx *c, bool half)
 {
     const int n[3] = {c->args.n[0], c->args.n[1], c->args.n[2]};
     const int in0[3] = {n[0] + 6, n[1] + 6, n[2] + 6}, in1[3] = {n[0] + 4, n[1] + 4, n[2] + 4},
@@ -267,7 +268,7 @@ int infer(dg_cnn *m, dg_ctx *c)
     void *X0 = ws(m, WS_X0, (size_t)in0[0] * in0[1] * in0[2] * 8 * 2);
     void *X1 = ws(m, WS_IX1, (size_t)in1[0] * in1[1] * in1[2] * 32 * 2);
     void *X2 = ws(m, WS_IX2, (size_t)in2[0] * in2[1] * in2[2] * 32 * 2);
-    float *G = (float *)ws(m, WS_IG, (size_t)n[0] * n[1] * n[2] * 16 * 4);
+    void *G = ws(m, WS_IG, (size_t)n[0] * n[1] * n[2] * 16 * 4);
     if (!X0 || !X1 || !X2 || !G) return mfail(m, DG_ERR_ALLOC, "workspace");
     dg::cnn::PrepArgs a = prep_args(m, 1, n, 0, 3, c->cfg.ds, c->cfg.gamma);
     for (int d = 0; d < 3; ++d) a.dims[d] = in0[d];
@@ -276,8 +277,12 @@ int infer(dg_cnn *m, dg_ctx *c)
                           m->stream), "conv1");
     MCHK(m, dg::cnn::conv(2, dg::cnn::CONV_INFER, 1, in1, X1, m->Wp[1], m->bias_pad[1], nullptr, X2, nullptr, nullptr,
                           m->stream), "conv2");
-    MCHK(m, dg::cnn::conv(3, dg::cnn::CONV_OUT, 1, in2, X2, m->Wp[2], m->bias_pad[2], nullptr, nullptr, G, nullptr,
-                          m->stream), "conv3");
+    if (half)
+        MCHK(m, dg::cnn::conv(3, dg::cnn::CONV_OUT16, 1, in2, X2, m->Wp[2], m->bias_pad[2], nullptr, G, nullptr,
+                              nullptr, m->stream), "conv3");
+    else
+        MCHK(m, dg::cnn::conv(3, dg::cnn::CONV_OUT, 1, in2, X2, m->Wp[2], m->bias_pad[2], nullptr, nullptr,
+                              static_cast<float *>(G), nullptr, m->stream), "conv3");
     return DG_OK;
 }
 
@@ -328,8 +333,8 @@ int prepare_infer(dg_cnn *m, dg_ctx *c)
     return DG_OK;
 }
 
-// Plain step, then the correction in WS_IG, then dt and clock.
-int correct_and_finalize(dg_cnn *m, dg_ctx *c)
+// Plain step, then the correction in WS_IG (bf16 when half), then dt and clock.
+int correct_and_finalize(dg_cnn *m, dg_ctx *c, bool half)
 {
     const void *in[5];
     void *out[5];
@@ -338,7 +343,7 @@ int correct_and_finalize(dg_cnn *m, dg_ctx *c)
         out[q] = c->buf[1 - c->cur][q];
     }
     MCHK(m, dg::launch_step(c->prec, c->fast, c->tile_y, c->args, in, out, c->sc, c->partials, m->stream), "step");
-    MCHK(m, dg::launch_corr(c->fast, c->args, out, (const float *)m->ws[WS_IG].p, m->norm + 10, c->sc, m->flags,
+    MCHK(m, dg::launch_corr(c->fast, c->args, out, m->ws[WS_IG].p, half ? 1 : 0, m->norm + 10, c->sc, m->flags,
                             c->partials, m->stream), "correction");
     MCHK(m, dg::launch_finalize(c->prec, c->args, c->sc, c->partials, c->nparts_reduce, c->cfg.cfl,
                                 c->cfg.dt_nan_fallback, 1, m->stream), "finalize");
@@ -350,9 +355,9 @@ int correct_and_finalize(dg_cnn *m, dg_ctx *c)
 // One learned step on the model's stream (the context's buffers and scalars).
 int learned_launch(dg_cnn *m, dg_ctx *c)
 {
-    int rc = infer(m, c);
+    int rc = infer(m, c, true);
     if (rc) return rc;
-    return correct_and_finalize(m, c);
+    return correct_and_finalize(m, c, true);
 }
 
 __global__ void arm_run_kernel(dg::Scalars *sc, double t_end, long long max_steps)
@@ -883,7 +888,7 @@ int dg_learned_step_given(dg_cnn *m, dg_ctx *c, const float *G, double *dt_used)
     hipLaunchKernelGGL(arm_run_kernel, dim3(1), dim3(1), 0, m->stream, c->sc, (double)INFINITY, (long long)LLONG_MAX);
     dg::Scalars s;
     MHIP(m, hipMemcpyAsync(&s, c->sc, sizeof(s), hipMemcpyDeviceToHost, m->stream));
-    if ((rc = correct_and_finalize(m, c))) return rc;
+    if ((rc = correct_and_finalize(m, c, false))) return rc;
     MHIP(m, hipStreamSynchronize(m->stream));
     if (dt_used) *dt_used = s.dt;
     return DG_OK;
@@ -914,8 +919,8 @@ int dg_learned_time(dg_cnn *m, dg_ctx *c, int nrep, double ms[5])
         MCHK(m, dg::cnn::conv(2, dg::cnn::CONV_INFER, 1, in1, m->ws[WS_IX1].p, m->Wp[1], m->bias_pad[1], nullptr,
                               m->ws[WS_IX2].p, nullptr, nullptr, m->stream), "conv2");
         MHIP(m, hipEventRecord(ev[3], m->stream));
-        MCHK(m, dg::cnn::conv(3, dg::cnn::CONV_OUT, 1, in2, m->ws[WS_IX2].p, m->Wp[2], m->bias_pad[2], nullptr,
-                              nullptr, (float *)m->ws[WS_IG].p, nullptr, m->stream), "conv3");
+        MCHK(m, dg::cnn::conv(3, dg::cnn::CONV_OUT16, 1, in2, m->ws[WS_IX2].p, m->Wp[2], m->bias_pad[2], nullptr,
+                              m->ws[WS_IG].p, nullptr, nullptr, m->stream), "conv3");   // as the learned step
         MHIP(m, hipEventRecord(ev[4], m->stream));
         MHIP(m, hipEventSynchronize(ev[4]));
         float t;
@@ -948,7 +953,7 @@ int dg_learned_correction(dg_cnn *m, dg_ctx *c, float *G)
     MHIP(m, hipSetDevice(m->device));
     int rc = prepare_infer(m, c);
     if (rc) return rc;
-    if ((rc = infer(m, c))) return rc;
+    if ((rc = infer(m, c, false))) return rc;
     MHIP(m, hipMemcpyAsync(G, m->ws[WS_IG].p, c->n3 * 16 * 4, hipMemcpyDeviceToHost, m->stream));
     MHIP(m, hipStreamSynchronize(m->stream));
     return DG_OK;

@@ -1196,8 +1196,13 @@ __device__ __forceinline__ long long wrap_index(int x, int y, int z, const StepA
     return ((long long)x * A.n[1] + y) * A.n[2] + z;
 }
 
+// G holds fp32 (host-given corrections) or bf16 (the network's output layer).
+__device__ __forceinline__ float gval(const float *G, long long i) { return G[i]; }
+__device__ __forceinline__ float gval(const __bf16 *G, long long i) { return (float)G[i]; }
+
 // Corrected increment of cell (x, y, z); flags == nullptr: all faces corrected.
-__device__ __forceinline__ void corr_delta(const StepArgs &A, const float *__restrict__ G, const CorrScale &S,
+template <typename GT>
+__device__ __forceinline__ void corr_delta(const StepArgs &A, const GT *__restrict__ G, const CorrScale &S,
                                            float coef, const unsigned char *__restrict__ flags, int x, int y, int z,
                                            float d[5])
 {
@@ -1215,7 +1220,7 @@ __device__ __forceinline__ void corr_delta(const StepArgs &A, const float *__res
     for (int q = 0; q < 5; ++q) {
         float div = 0.0f;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) div += mh[a] * G[c * 16 + 5 * a + q] - ml[a] * G[lo[a] * 16 + 5 * a + q];
+        for (int a = 0; a < 3; ++a) div += mh[a] * gval(G, c * 16 + 5 * a + q) - ml[a] * gval(G, lo[a] * 16 + 5 * a + q);
         d[q] = coef * S.s[q] * div;
     }
 }
@@ -1230,9 +1235,9 @@ __global__ void zero_count_kernel(int *cnt) { *cnt = 0; }
 // MASKED = true (skipped when cnt is 0; U and partials untouched):
 // flags[i] = prev[i] or the plain state B of i plus its increment with the
 // faces of the cells flagged in prev kept plain is invalid.
-template <bool FAST, bool MASKED>
+template <bool FAST, bool MASKED, typename GT>
 __global__ void __launch_bounds__(256)
-corr_flag_kernel(const StepArgs A, const Fields<float> U, float *__restrict__ B, const float *__restrict__ G,
+corr_flag_kernel(const StepArgs A, const Fields<float> U, float *__restrict__ B, const GT *__restrict__ G,
                  const CorrScale S, const Scalars *__restrict__ sc, const unsigned char *__restrict__ prev,
                  unsigned char *__restrict__ flags, int *__restrict__ cnt, double *__restrict__ partials)
 {
@@ -1273,9 +1278,9 @@ corr_flag_kernel(const StepArgs A, const Fields<float> U, float *__restrict__ B,
 // Only when a cell is flagged (cnt > 0): U = B + the increments under the
 // final flags, and the wave-speed partials of that state.  With no cell
 // flagged the flag kernel's corrected state and partials stand.
-template <bool FAST>
+template <bool FAST, typename GT>
 __global__ void __launch_bounds__(256)
-corr_apply_kernel(const StepArgs A, const Fields<float> U, const float *__restrict__ B, const float *__restrict__ G,
+corr_apply_kernel(const StepArgs A, const Fields<float> U, const float *__restrict__ B, const GT *__restrict__ G,
                   const CorrScale S, const Scalars *__restrict__ sc, const unsigned char *__restrict__ flags,
                   const int *__restrict__ cnt, double *__restrict__ partials)
 {
@@ -1590,32 +1595,44 @@ static int cells_grid(long long n3)
 static size_t corr_b_offset(long long n3) { return (size_t)((2 * n3 + 63) / 64) * 64; }
 size_t corr_flags_bytes(long long n3) { return corr_b_offset(n3) + (size_t)n3 * 5 * sizeof(float) + 64; }
 
-int launch_corr(int fast, const StepArgs &a, void *const u[5], const float *G, const float scale[5], const Scalars *sc,
-                unsigned char *flags, double *partials, hipStream_t s)
+template <typename GT>
+static void launch_corr_t(int fast, const StepArgs &a, const Fields<float> &U, float *B, const GT *G,
+                          const CorrScale &S, const Scalars *sc, unsigned char *fa, unsigned char *fb, int *cnt,
+                          double *partials, hipStream_t s)
+{
+    const long long n3 = (long long)a.n[0] * a.n[1] * a.n[2];
+    const int nb = reduce_blocks(a);   // the flag and apply kernels write the finalize's partials
+    const dim3 g(cells_grid(n3)), b(256);
+    if (fast) {
+        hipLaunchKernelGGL((corr_flag_kernel<true, false, GT>), dim3(nb), b, 0, s, a, U, B, G, S, sc, nullptr, fa, cnt,
+                           partials);
+        hipLaunchKernelGGL((corr_flag_kernel<true, true, GT>), g, b, 0, s, a, U, B, G, S, sc, fa, fb, cnt, partials);
+        hipLaunchKernelGGL((corr_flag_kernel<true, true, GT>), g, b, 0, s, a, U, B, G, S, sc, fb, fa, cnt, partials);
+        hipLaunchKernelGGL((corr_apply_kernel<true, GT>), dim3(nb), b, 0, s, a, U, B, G, S, sc, fa, cnt, partials);
+    } else {
+        hipLaunchKernelGGL((corr_flag_kernel<false, false, GT>), dim3(nb), b, 0, s, a, U, B, G, S, sc, nullptr, fa,
+                           cnt, partials);
+        hipLaunchKernelGGL((corr_flag_kernel<false, true, GT>), g, b, 0, s, a, U, B, G, S, sc, fa, fb, cnt, partials);
+        hipLaunchKernelGGL((corr_flag_kernel<false, true, GT>), g, b, 0, s, a, U, B, G, S, sc, fb, fa, cnt, partials);
+        hipLaunchKernelGGL((corr_apply_kernel<false, GT>), dim3(nb), b, 0, s, a, U, B, G, S, sc, fa, cnt, partials);
+    }
+}
+
+int launch_corr(int fast, const StepArgs &a, void *const u[5], const void *G, int g_bf16, const float scale[5],
+                const Scalars *sc, unsigned char *flags, double *partials, hipStream_t s)
 {
     CorrScale S;
     for (int q = 0; q < 5; ++q) S.s[q] = scale[q];
-    const int nb = reduce_blocks(a);   // the flag and apply kernels write the finalize's partials
     const long long n3 = (long long)a.n[0] * a.n[1] * a.n[2];
     unsigned char *fa = flags, *fb = flags + n3;
     float *B = reinterpret_cast<float *>(flags + corr_b_offset(n3));
     int *cnt = reinterpret_cast<int *>(flags + corr_flags_bytes(n3) - 64);
     const Fields<float> U = mfields<float>(u);
-    const dim3 g(cells_grid(n3)), b(256);
     hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, s, cnt);
-    if (fast) {
-        hipLaunchKernelGGL((corr_flag_kernel<true, false>), dim3(nb), b, 0, s, a, U, B, G, S, sc, nullptr, fa, cnt,
-                           partials);
-        hipLaunchKernelGGL((corr_flag_kernel<true, true>), g, b, 0, s, a, U, B, G, S, sc, fa, fb, cnt, partials);
-        hipLaunchKernelGGL((corr_flag_kernel<true, true>), g, b, 0, s, a, U, B, G, S, sc, fb, fa, cnt, partials);
-        hipLaunchKernelGGL((corr_apply_kernel<true>), dim3(nb), b, 0, s, a, U, B, G, S, sc, fa, cnt, partials);
-    } else {
-        hipLaunchKernelGGL((corr_flag_kernel<false, false>), dim3(nb), b, 0, s, a, U, B, G, S, sc, nullptr, fa, cnt,
-                           partials);
-        hipLaunchKernelGGL((corr_flag_kernel<false, true>), g, b, 0, s, a, U, B, G, S, sc, fa, fb, cnt, partials);
-        hipLaunchKernelGGL((corr_flag_kernel<false, true>), g, b, 0, s, a, U, B, G, S, sc, fb, fa, cnt, partials);
-        hipLaunchKernelGGL((corr_apply_kernel<false>), dim3(nb), b, 0, s, a, U, B, G, S, sc, fa, cnt, partials);
-    }
+    if (g_bf16)
+        launch_corr_t<__bf16>(fast, a, U, B, static_cast<const __bf16 *>(G), S, sc, fa, fb, cnt, partials, s);
+    else
+        launch_corr_t<float>(fast, a, U, B, static_cast<const float *>(G), S, sc, fa, fb, cnt, partials, s);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -258,6 +258,31 @@ def test_learned_correction_fallback_reaches_the_neighbours(emu_lib):
     a.close(), p.close()
 
 
+def to_bf16(x):
+    """float32 -> nearest bfloat16 (ties to even), returned as float32."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32)
+    u = (u + (((u >> 16) & 1) + 0x7FFF)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def test_learned_step_applies_the_bf16_correction(emu_lib):
+    """A learned step applies the output layer rounded to bfloat16 (nearest,
+    ties to even): bit for bit a host-given correction made of
+    dg_learned_correction's fp32 output rounded so, from the same state."""
+    m = model(emu_lib, seed=10)
+    a, b = periodic_solver(emu_lib, 6, 4), periodic_solver(emu_lib, 6, 4)
+    G = m.correction(b)
+    Gh = to_bf16(G)
+    assert not np.array_equal(Gh, G) and np.abs(Gh - G).max() <= 2.0 ** -8 * np.abs(G).max()
+    assert to_bf16(np.float32([1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8]))[0] == 1.0   # ties to even
+    dt_b = m.step_given(b, Gh)
+    dt_a = m.step(a)
+    assert dt_a == dt_b
+    np.testing.assert_array_equal(interior(a.get_state()), interior(b.get_state()))
+    assert a.next_dt() == b.next_dt()
+    a.close(), b.close()
+
+
 def test_learned_inputs_are_normalized_primitives(emu_lib):
     """dg_learned_correction: the network applied to the normalized
     primitives of the periodic state (halo of 3 wrapped cells)."""

@@ -165,6 +165,24 @@ def test_correction_fallback_and_conservation():
     np.testing.assert_allclose(sums1, sums0, rtol=0, atol=1e-5 * np.abs(Up).sum(axis=(1, 2, 3)).max())
 
 
+def test_learned_step_applies_the_bf16_correction():
+    """A learned step applies the output layer rounded to bfloat16 (nearest,
+    ties to even): bit for bit a host-given correction made of
+    dg_learned_correction's fp32 output rounded so, from the same state."""
+    from test_cnn_emulation import to_bf16
+    m = model(seed=10)
+    a, b = solver(24, 4), solver(24, 4)
+    G = m.correction(b)
+    Gh = to_bf16(G)
+    assert not np.array_equal(Gh, G)
+    dt_b = m.step_given(b, Gh)
+    dt_a = m.step(a)
+    assert dt_a == dt_b
+    np.testing.assert_array_equal(interior(a.get_state()), interior(b.get_state()))
+    assert a.next_dt() == b.next_dt()
+    a.close(), b.close()
+
+
 def test_learned_inputs_are_normalized_primitives():
     m = model(seed=8)
     norm = np.array([1.1, 0.0, 0.01, -0.02, 1.2, 0.2, 0.3, 0.3, 0.35, 0.15] + [1.0] * 5, np.float32)
