@@ -49,7 +49,10 @@ int pack(int layer, int transpose, const float *W, void *Wp, hipStream_t s);
 int pack_elems(int layer, int transpose);
 int adam(float *w, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps, long long t,
          hipStream_t s);
-int prep_input(const PrepArgs &a, const float *const *samples, const int *boxes, void *X, hipStream_t s);
+// zero (optional): a counter the kernel also sets to 0 (the learned step's
+// fallback count, which saves that step a launch).
+int prep_input(const PrepArgs &a, const float *const *samples, const int *boxes, void *X, hipStream_t s,
+               int *zero = nullptr);
 int prep_target(const PrepArgs &a, const float *const *samples, const int *boxes, float *T, hipStream_t s);
 int loss(int B, int L, const float *G, const float *T, void *dG, double *loss_part, int nblk, hipStream_t s);
 int to_bf16(const float *x, long long ncells, int creal, int cpad, void *y, hipStream_t s);

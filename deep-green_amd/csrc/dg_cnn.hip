@@ -600,8 +600,9 @@ __global__ void adam_kernel(float *__restrict__ w, const float *__restrict__ gr,
 // one box per batch item: boxes[b] = (sample, o0, o1, o2).
 __global__ void __launch_bounds__(256)
 prep_input_kernel(const PrepArgs a, const float *const *__restrict__ samples, const int *__restrict__ boxes,
-                  bf16 *__restrict__ X)
+                  bf16 *__restrict__ X, int *__restrict__ zero)
 {
+    if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
     const long long per = (long long)a.dims[0] * a.dims[1] * a.dims[2];
     const long long total = per * a.B;
     const float V = a.ds * a.ds * a.ds, gm1 = a.gamma - 1.0f;
@@ -885,11 +886,11 @@ int adam(float *w, const float *g, float *m, float *v, int n, float lr, float b1
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int prep_input(const PrepArgs &a, const float *const *samples, const int *boxes, void *X, hipStream_t s)
+int prep_input(const PrepArgs &a, const float *const *samples, const int *boxes, void *X, hipStream_t s, int *zero)
 {
     const long long n = (long long)a.B * a.dims[0] * a.dims[1] * a.dims[2];
     hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, a, samples, boxes,
-                       static_cast<bf16 *>(X));
+                       static_cast<bf16 *>(X), zero);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

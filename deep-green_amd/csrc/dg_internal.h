@@ -104,8 +104,11 @@ int launch_ens_finalize(int prec, const EnsInstance *inst, int ninst, const doub
 // channels, fp32 (g_bf16 = 0) or bf16 (g_bf16 = 1).  flags has
 // corr_flags_bytes(n3) bytes; its first n3 hold the final flags.
 size_t corr_flags_bytes(long long n3);
+// count_zeroed: the fallback count (corr_count) was already set to 0 on the
+// stream since its last use; otherwise launch_corr clears it.
 int launch_corr(int fast, const StepArgs &a, void *const u[5], const void *G, int g_bf16, const float scale[5],
-                const Scalars *sc, unsigned char *flags, double *partials, hipStream_t s);
+                const Scalars *sc, unsigned char *flags, double *partials, hipStream_t s, int count_zeroed = 0);
+int *corr_count(unsigned char *flags, long long n3);
 // FP32: coarse = 2x restriction of fine (sum of the 8 children).
 int launch_restrict(const StepArgs &fine, const void *const f[5], const StepArgs &coarse, void *const c[5], hipStream_t s);
 // FP32: r = (ut - up) / (dt ds^2).

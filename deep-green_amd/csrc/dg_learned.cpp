@@ -272,7 +272,9 @@ int infer(dg_cnn *m, dg_ctx *c, bool half)
     if (!X0 || !X1 || !X2 || !G) return mfail(m, DG_ERR_ALLOC, "workspace");
     dg::cnn::PrepArgs a = prep_args(m, 1, n, 0, 3, c->cfg.ds, c->cfg.gamma);
     for (int d = 0; d < 3; ++d) a.dims[d] = in0[d];
-    MCHK(m, dg::cnn::prep_input(a, m->d_ptrs + 5 * c->cur, m->d_box0, X0, m->stream), "prep");
+    // the learned step's forward also clears the correction's fallback count
+    MCHK(m, dg::cnn::prep_input(a, m->d_ptrs + 5 * c->cur, m->d_box0, X0, m->stream,
+                                half ? dg::corr_count(m->flags, c->n3) : nullptr), "prep");
     MCHK(m, dg::cnn::conv(1, dg::cnn::CONV_INFER, 1, in0, X0, m->Wp[0], m->bias_pad[0], nullptr, X1, nullptr, nullptr,
                           m->stream), "conv1");
     MCHK(m, dg::cnn::conv(2, dg::cnn::CONV_INFER, 1, in1, X1, m->Wp[1], m->bias_pad[1], nullptr, X2, nullptr, nullptr,
@@ -343,8 +345,9 @@ int correct_and_finalize(dg_cnn *m, dg_ctx *c, bool half)
         out[q] = c->buf[1 - c->cur][q];
     }
     MCHK(m, dg::launch_step(c->prec, c->fast, c->tile_y, c->args, in, out, c->sc, c->partials, m->stream), "step");
+    // half: called after infer(half), whose input preparation cleared the fallback count
     MCHK(m, dg::launch_corr(c->fast, c->args, out, m->ws[WS_IG].p, half ? 1 : 0, m->norm + 10, c->sc, m->flags,
-                            c->partials, m->stream), "correction");
+                            c->partials, m->stream, half ? 1 : 0), "correction");
     MCHK(m, dg::launch_finalize(c->prec, c->args, c->sc, c->partials, c->nparts_reduce, c->cfg.cfl,
                                 c->cfg.dt_nan_fallback, 1, m->stream), "finalize");
     c->cur = 1 - c->cur;

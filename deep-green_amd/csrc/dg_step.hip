@@ -1618,17 +1618,22 @@ static void launch_corr_t(int fast, const StepArgs &a, const Fields<float> &U, f
     }
 }
 
+int *corr_count(unsigned char *flags, long long n3)
+{
+    return reinterpret_cast<int *>(flags + corr_flags_bytes(n3) - 64);
+}
+
 int launch_corr(int fast, const StepArgs &a, void *const u[5], const void *G, int g_bf16, const float scale[5],
-                const Scalars *sc, unsigned char *flags, double *partials, hipStream_t s)
+                const Scalars *sc, unsigned char *flags, double *partials, hipStream_t s, int count_zeroed)
 {
     CorrScale S;
     for (int q = 0; q < 5; ++q) S.s[q] = scale[q];
     const long long n3 = (long long)a.n[0] * a.n[1] * a.n[2];
     unsigned char *fa = flags, *fb = flags + n3;
     float *B = reinterpret_cast<float *>(flags + corr_b_offset(n3));
-    int *cnt = reinterpret_cast<int *>(flags + corr_flags_bytes(n3) - 64);
+    int *cnt = corr_count(flags, n3);
     const Fields<float> U = mfields<float>(u);
-    hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, s, cnt);
+    if (!count_zeroed) hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, s, cnt);
     if (g_bf16)
         launch_corr_t<__bf16>(fast, a, U, B, static_cast<const __bf16 *>(G), S, sc, fa, fb, cnt, partials, s);
     else
