@@ -285,6 +285,10 @@ int make_args(const dg_config *cfg, StepArgs *out, int *tile_y_out, int *heat_ma
     if (cfg->precision < 0 || cfg->precision > 2 || cfg->math < 0 || cfg->math > 1)
         return bad("unknown precision or math");
     if (cfg->limiter < 0 || cfg->limiter > 2) return bad("unknown limiter");
+    // fast math keeps extensive (times ds^3) face states; in FP32 their
+    // products stay normal, finite numbers for 1e-5 <= ds <= 1e4
+    if (cfg->math == DG_MATH_FAST && cfg->precision != DG_FP64 && !(cfg->ds >= 1e-5 && cfg->ds <= 1e4))
+        return bad("fast math in FP32 needs 1e-5 <= ds <= 1e4 (exact math has no such limit)");
     const size_t n3 = (size_t)cfg->n[0] * cfg->n[1] * cfg->n[2];
     // the kernels address a field with 32-bit byte offsets
     const size_t el_bytes = cfg->precision == DG_FP64 ? sizeof(double) : sizeof(float);

@@ -311,7 +311,7 @@ __device__ __forceinline__ int rot_field(int axis, int slot)
 template <typename T>
 struct Consts {
     T ds, gamma, gm1, V, invV, two_ds, inv_two_ds, inv_ds, half_ds, half_dt, dtds, G;
-    T hgm1, sK, irk;   // fast math: (gamma - 1) / 2, sqrt(gamma (gamma - 1) / 2), (gamma - 1) V / 2
+    T hgm1, sK, irk;   // fast math: (gamma - 1) / 2, sqrt(gamma (gamma - 1) / 2), (gamma - 1) / 2
     bool pow2;   // ds a power of two: divisions by ds, 2 ds, ds^3 as exact products
 };
 
@@ -335,10 +335,12 @@ __device__ __forceinline__ Consts<T> make_consts(const StepArgs &A, T dt)
     C.half_ds = FAST ? (one_sided ? T(0.5) : T(0.25)) : C.ds / T(2.);
     C.G = T(A.g);
     C.half_dt = FAST ? T(0.5) * dt * (one_sided ? C.inv_ds : C.inv_two_ds) : T(0.5) * dt;
-    C.dtds = FAST ? dt * C.ds * C.ds * T(0.5) : dt * C.ds;
+    // fast math: the ring and the fluxes are extensive (times V), so the
+    // update divides by V: dt ds^2 / 2 / ds^3
+    C.dtds = FAST ? T(0.5) * dt * C.inv_ds : dt * C.ds;
     C.hgm1 = T(0.5) * C.gm1;
     C.sK = dsqrt<T>(C.gamma * C.hgm1);
-    C.irk = C.hgm1 * C.V;
+    C.irk = C.hgm1;
     return C;
 }
 
@@ -368,12 +370,16 @@ struct StepLDS {
 };
 
 // The six ring words of a cell from its conserved state u.  Exact math:
-// calc_primitives (rho, vx, vy, vz, p) and a pad.  Fast math: the pressure
-// word holds P = 2 p / (gamma - 1) = (2 E - |m|^2 / U_rho) / V, which the
-// flux uses as is (twice the internal energy density) and the Hancock step
-// extrapolates like p (it is linear in p), and the sixth word
-// 1 / (k rho) = irk / U_rho with k = 2 / (gamma - 1), so that the Hancock
-// step's grad(p) / rho is that word times grad(P).
+// calc_primitives (rho, vx, vy, vz, p) and a pad.  Fast math, extensive
+// (times V, so no division by V): the mass U_rho, the velocities, the
+// pressure word P = 2 E - |m|^2 / U_rho = V 2 p / (gamma - 1), which the flux
+// uses as is (twice the internal energy) and the Hancock step extrapolates
+// like p (it is linear in p), and the sixth word 1 / (k U_rho) = irk / U_rho
+// with k = 2 / (gamma - 1), so that the Hancock step's grad(p) / rho is that
+// word times grad(P).  Every face state, and so every flux, is then V times
+// the intensive one; the sound speed sqrt(gamma (gamma - 1) / 2) |P| /
+// sqrt(P U_rho) does not change.  FP32 keeps P U_rho ~ V^2 normal and
+// finite for 1e-5 <= ds <= 1e4 (checked at create).
 template <typename T, bool FAST>
 __device__ __forceinline__ void ring_cell(const T u[5], const Consts<T> &C, T w[6])
 {
@@ -383,9 +389,9 @@ __device__ __forceinline__ void ring_cell(const T u[5], const Consts<T> &C, T w[
     } else {
         const T im = drcp<T>(u[0]);
         const T ux = u[1] * im, uy = u[2] * im, uz = u[3] * im;
-        w[0] = u[0] * C.invV; w[1] = ux; w[2] = uy; w[3] = uz;
+        w[0] = u[0]; w[1] = ux; w[2] = uy; w[3] = uz;
         const T ke2 = dfma<T>(uz, u[3], dfma<T>(uy, u[2], ux * u[1]));   // |m|^2 / U_rho
-        w[4] = dfma<T>(T(2), u[4], -ke2) * C.invV;
+        w[4] = dfma<T>(T(2), u[4], -ke2);
         w[5] = C.irk * im;
     }
 }
@@ -1411,7 +1417,8 @@ heat_kernel(const StepArgs A, const Fields<const T> Uold, const Fields<T> Unew, 
         if (A.limiter == LIM_MINMOD) cell_xtr<T, FAST, LIM_MINMOD>(A, Uold, C, i, j, k, xtr);
         else if (A.limiter == LIM_VANLEER) cell_xtr<T, FAST, LIM_VANLEER>(A, Uold, C, i, j, k, xtr);
         else cell_xtr<T, FAST, LIM_NONE>(A, Uold, C, i, j, k, xtr);
-        const T p_xtr = FAST ? xtr[4] * C.hgm1 : xtr[4];   // fast math: the ring's 2 p / (gamma - 1)
+        // fast math: the ring's V 2 p / (gamma - 1) and V rho (the V cancels)
+        const T p_xtr = FAST ? xtr[4] * C.hgm1 : xtr[4];
         const T t_in = p_xtr / (T(H.rs) * xtr[0]);
         const T Q = (T(H.t_out) - t_in) * (S / (dk + T(1.0) / hc));
         const T dE = dt * ds * ((Q / V) * ds);

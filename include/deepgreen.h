@@ -51,7 +51,8 @@ typedef enum dg_precision {
 typedef enum dg_math {
     DG_MATH_EXACT = 0, /* the reference's operation order, IEEE div/sqrt, no FMA */
     DG_MATH_FAST = 1   /* same scheme, reassociated with FMA, hardware reciprocals and
-                          reciprocal square roots (FP64: refined by Newton steps) */
+                          reciprocal square roots (FP64: refined by Newton steps);
+                          FP32 and FP32-bulk need 1e-5 <= ds <= 1e4 */
 } dg_math;
 
 /* Slope used by the MUSCL-Hancock reconstruction.  NONE is the reference's

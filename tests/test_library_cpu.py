@@ -43,6 +43,8 @@ def test_config_default_is_reference_constants():
     lambda c: setattr(c, "precision", 5),
     lambda c: setattr(c, "limiter", 9),
     lambda c: setattr(c, "tile_y", 5),
+    lambda c: (setattr(c, "math", 1), setattr(c, "ds", 1e-6)),   # fast FP32 outside its ds range
+    lambda c: (setattr(c, "math", 1), setattr(c, "ds", 1e5)),
 ])
 def test_invalid_configs_are_rejected_before_device_use(mutate):
     import deepgreen
